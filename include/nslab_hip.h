@@ -1,0 +1,136 @@
+/*
+ * nslab_hip.h — C ABI of the MI355X (gfx950) NSLab solver core: libnslab_hip.so.
+ *
+ * Plain C types only (no HIP or torch types). Every entry point returns NSL_OK (0) or a negative status; the text of
+ * the last failure on the calling thread is returned by nsl_last_error(). No exception crosses this boundary.
+ * The Node host (nslab.js in the package) binds these through a hand-built N-API addon; Python tests bind them
+ * through ctypes.
+ *
+ * Citations name the reference interface each entry point replaces (pocket-wind-tunnel/src/nslab.js = "nslab.js").
+ * Spectral arrays at this boundary use the reference layout: six float64 planes (ux re, ux im, uy re, uy im, uz re,
+ * uz im), each of NS = N*N*(N/2+1) values indexed (kz*N + ky)*(N/2+1) + kx (nslab.js:94, :184).
+ */
+#ifndef NSLAB_HIP_H
+#define NSLAB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSL_OK 0
+#define NSL_ERR_ARG -1      /* bad argument: grid size, unknown IC, null pointer (message as the reference's throw) */
+#define NSL_ERR_HIP -2      /* HIP runtime failure (allocation, launch, copy) */
+#define NSL_ERR_STATE -3    /* call not valid in the solver's current state */
+
+/* Initial conditions (nslab.js:161-169 IC_INFO, setIC :250-292). */
+#define NSL_IC_TGV 0
+#define NSL_IC_TGV2D 1
+#define NSL_IC_ABC 2
+#define NSL_IC_TUBES 3
+#define NSL_IC_RANDOM 4
+
+typedef struct nsl_solver nsl_solver;
+
+/* createSolver(opts) options (nslab.js:171-175). The caller applies the reference's defaults: nu = 1/(Re || 100),
+ * cfl || 0.4, dt || 0 (0 = adaptive). icp: tubes {amp, sigma, sep, pert}; abc {A, B, C, -}; random {seed, k0, E0, -};
+ * tgv/tgv2d ignore it. */
+typedef struct {
+  int32_t N;
+  int32_t ic;
+  double nu;
+  double cfl;
+  double dt;
+  double icp[4];
+} nsl_opts;
+
+/* solver.last — diagnostics of the most recent rhs() input state (nslab.js:246). */
+typedef struct {
+  double E, Z, P, eps, Tnl, Pspec, omMax, uMax;
+} nsl_rhsdiag;
+
+/* One RK4 step as step() leaves it (nslab.js:312-333): st.t, st.dt, st.step after the step, the two budget
+ * residuals, `last` (the rhs of the new state, the values record() appends) and d0 (the rhs of the state the step
+ * started from, whose |Tnl|/eps feeds st.maxTnl). */
+typedef struct {
+  double t, dt;
+  int64_t step;
+  double ebal, zbal;
+  nsl_rhsdiag last;
+  nsl_rhsdiag d0;
+} nsl_steprec;
+
+/* createSolver(opts) (nslab.js:171): allocates the device state, builds the IC and runs the first rhs.
+ * On success *out is a new solver; st.E0/st.Z0 are last.E/last.Z of the returned nsl_last(). */
+int nsl_create(const nsl_opts *opts, nsl_solver **out);
+/* solver.setIC(kind, prm) (nslab.js:250): resets t and step to 0 and recomputes last. */
+int nsl_set_ic(nsl_solver *s, int32_t ic, const double icp[4]);
+void nsl_destroy(nsl_solver *s);
+
+/* run(nSteps, tEnd) (nslab.js:334): up to n steps, stopping before a step once t >= tEnd - 1e-12 (has_tend != 0).
+ * recs (may be NULL) receives one nsl_steprec per step taken; *n_done the count. */
+int nsl_run(nsl_solver *s, int64_t n, int32_t has_tend, double tend, nsl_steprec *recs, int64_t *n_done);
+
+/* Clock the step loop reads: st.t, st.step and st.E0 (the energy the budget residual is normalised by). Callers
+ * that assign st (checkpoint restore, nslab-worker) push the new values here before stepping. */
+int nsl_set_clock(nsl_solver *s, double t, int64_t step, double E0);
+int nsl_get_clock(nsl_solver *s, double *t, int64_t *step, double *dt);
+
+/* solver.last (nslab.js:648 getter). */
+int nsl_last(nsl_solver *s, nsl_rhsdiag *out);
+/* solver.norms() (nslab.js:214-222): out = {E, Z, P}. */
+int nsl_norms(nsl_solver *s, double out[3]);
+
+/* solver._S (nslab.js:184, 648): copy the state out to / in from six host planes of NS float64 each. Modes outside
+ * the 2/3 mask are identically zero in every state the solver produces (nslab.js:235, :289) and are not stored;
+ * set_state drops them. set_state re-evaluates rhs so `last` describes the new state, as step() would at entry. */
+int nsl_get_state(nsl_solver *s, double *const planes[6]);
+int nsl_set_state(nsl_solver *s, const double *const planes[6]);
+
+/* solver.diagnose() (nslab.js:339-392). out receives NSL_DIAG_LEN + nb doubles (nb = nsl_spectrum_len):
+ *  0 t, 1 step, 2 E, 3 Z, 4 P, 5 eps, 6 omMax, 7 uMax, 8 Pspec, 9 Pphys, 10 Tnl, 11 skew, 12 divMax, 13 eta,
+ *  14 kmaxEta, 15 Rel, 16 lambda, 17 skewIso, 18..20 align, 21 uL3, 22 pileUp (NaN = null), 23 zCentroid,
+ *  24 zExtent, 25 imageGap, 26 zBands, 27 omMaxI (NaN = not run), 28..30 omMaxIpos, 31 omMaxNode,
+ *  32..79 alignHist[3][16], 80.. spectrum[nb].
+ * do_interp != 0 runs interpMax when N <= 128 (nslab.js:389-390). Updates the solver's divMax (returned by
+ * nsl_get_divmax). */
+#define NSL_DIAG_LEN 80
+int nsl_diagnose(nsl_solver *s, int32_t do_interp, double *out);
+int nsl_get_divmax(nsl_solver *s, double *divmax);
+int nsl_set_divmax(nsl_solver *s, double divmax);
+/* solver.spectrum() (nslab.js:414-420) and its length ceil(sqrt(3) kc) + 2. */
+int nsl_spectrum_len(nsl_solver *s);
+int nsl_spectrum(nsl_solver *s, double *Ek);
+/* solver.interpMax(top, iters) (nslab.js:421-536): out = {omMaxI, x, y, z, omMaxNode}. Needs the vorticity field of
+ * a preceding nsl_diagnose. */
+int nsl_interp_max(nsl_solver *s, int32_t top, int32_t iters, double out[5]);
+/* solver.fields / slice() source (nslab.js:431, 566-583): physical field `which` (0..2 u v w, 3..5 omega after
+ * diagnose or u x omega after a step, 6 Q, 7 stretching) as N^3 float64 in (z*N + y)*N + x order. */
+int nsl_field(nsl_solver *s, int32_t which, double *out);
+/* solver.exactError() (nslab.js:585-594): returns 1 and fills {linf, rel} for abc/tgv2d, 0 for other ICs. */
+int nsl_exact_error(nsl_solver *s, double out[2]);
+
+/* Bytes of device memory the solver holds (admission accounting). */
+int64_t nsl_device_bytes(const nsl_solver *s);
+
+/* NS.fft1 / NS.Spectral (nslab.js:46-155) on the GPU: `lines` independent length-N lines (split re/im, line-major)
+ * transformed in place, sign -1 forward, +1 inverse, unnormalised. */
+int nsl_fft1(int32_t N, int32_t sign, int64_t lines, double *re, double *im);
+/* Spectral.forward: real f[N^3] -> re, im [NS], normalised by 1/N^3 so f = sum u_k e^{ik.x}. */
+int nsl_spectral_forward(int32_t N, const double *f, double *re, double *im);
+/* Spectral.inverse: re, im [NS] -> real f[N^3], unnormalised. */
+int nsl_spectral_inverse(int32_t N, const double *re, const double *im, double *f);
+
+/* factorize(N) (nslab.js:28-35): radices into f[0..return), or NSL_ERR_ARG with the reference's message. */
+int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
+
+/* Text of the last error on this thread ("" if none). */
+const char *nsl_last_error(void);
+/* Version string of the library. */
+const char *nsl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
