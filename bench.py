@@ -1,0 +1,207 @@
+#!/usr/bin/env python
+"""Benchmark: FP64 RK4 steps/s of the NSLab pseudo-spectral Navier-Stokes solver on one MI355X.
+
+Workload (the headline configuration): Taylor-Green vortex on [0, 2pi]^3 at 256^3, nu = 1/1600, fixed dt = 0.004,
+FP64, 2/3 dealiasing, rotational form, classical RK4. A "step" is one RK4 step (4 right-hand sides). The state is
+resident in HBM when the timed region starts; the timed region is K steps between two device synchronisations.
+
+Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
+its mean launch time, from HIP events recorded on the solver's stream inside the timed region), the whole-step
+roofline fraction of the truncated-storage traffic model, a 512^3 line, and the CPU baseline (the oracle, a
+single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256] [--no-512] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+CLASSES = ["k_zinv", "k_yinv", "k_xnl", "k_yfwd", "k_zfwd", "k_update<1>", "k_update<2|3>", "k_update<4>", "k_finalize"]
+
+
+def geometry(N):
+    kc = N // 3
+    h, b = kc + 1, 2 * kc + 1
+    return h, b, b * b * h
+
+
+def alg_bytes_per_launch(N):
+    """Algorithmic HBM bytes per launch of each kernel class: every operand read once and written once
+    (complex FP64 = 16 B). h = floor(N/3)+1 retained kx, b = 2 floor(N/3)+1 retained ky/kz, D = b*b*h."""
+    h, b, D = geometry(N)
+    c = 16.0
+    return {
+        "k_zinv": (3 * D + 6 * h * b * N) * c,           # 3 velocity fields in, u and omega (6) out
+        "k_yinv": (6 * h * b * N + 6 * h * N * N) * c,   # 6 in, 6 out
+        "k_xnl": (6 * h * N * N + 3 * h * N * N) * c,    # 6 in, u x omega (3) out
+        "k_yfwd": (3 * h * N * N + 3 * h * b * N) * c,   # 3 in, 3 out
+        "k_zfwd": (3 * h * b * N + 3 * D) * c,           # 3 in, NL (3) out
+        "k_update<1>": 12 * D * c,                       # NL, S in; ACC, T out
+        "k_update<2|3>": 18 * D * c,                     # NL, T, S, ACC in; T, ACC out
+        "k_update<4>": 15 * D * c,                       # NL, T, S, ACC in; S out
+        "k_finalize": 0.0,
+    }
+
+
+def b_alg_step(N):
+    """Minimum HBM bytes of one RK4 step for a 5-pass-per-rhs schedule over 2/3-truncated storage:
+    B_alg(N) = 1152 h (b^2 + b N + N^2)."""
+    h, b, _ = geometry(N)
+    return 1152.0 * h * (b * b + b * N + N * N)
+
+
+def measure(N, steps, warmup, dt, label):
+    import capi
+    import torch
+    s = capi.Solver(N=N, Re=1600, ic="tgv", dt=dt)
+    s.run(warmup)
+    torch.cuda.synchronize()
+    L = capi.lib()
+    capi.check(L.nsl_profile(s.h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = s.run(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_double * len(CLASSES))()
+    cnt = (ctypes.c_int64 * len(CLASSES))()
+    capi.check(L.nsl_profile_read(s.h, ms, cnt))
+    capi.check(L.nsl_profile(s.h, 0))
+    assert done == steps, (done, steps)
+    elapsed = t1 - t0
+    per = alg_bytes_per_launch(N)
+    kern = {}
+    for i, name in enumerate(CLASSES):
+        if cnt[i]:
+            avg_s = ms[i] / cnt[i] / 1e3
+            kern[name] = {"launches": int(cnt[i]), "total_ms": ms[i], "avg_us": avg_s * 1e6,
+                          "alg_bytes": per[name], "GBs": per[name] / avg_s / 1e9 if per[name] else 0.0}
+    dom = max((k for k in kern if per[k] > 0), key=lambda k: kern[k]["total_ms"])
+    E = s.last["E"]
+    s.close()
+    del s
+    return {"label": label, "N": N, "steps": steps, "elapsed_s": elapsed, "steps_per_s": steps / elapsed,
+            "ms_per_step": 1e3 * elapsed / steps, "kernels": kern, "dominant": dom, "E_end": E,
+            "step_roofline_frac": b_alg_step(N) * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
+
+
+def cpu_baseline(N, dt, budget_s=20.0):
+    """The oracle (single-threaded C restatement of the reference algorithm, 5 rhs per step as nslab.js) timed on
+    this host: one untimed set-up, then RK4 steps until about budget_s seconds of CPU work (at most 3)."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    o = O.OracleSolver(N=N, Re=1600, ic="tgv", dt=dt)
+    t_setup = time.perf_counter() - t0
+    n, t0 = 0, time.perf_counter()
+    while n < 3:
+        o.run(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    del o
+    return {"value": n / el, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "TGV %d^3 FP64 dt %g: %d RK4 steps timed after an untimed set-up (%.1f s), %.1f s on 1 host "
+                      "thread; host %s, %d cores visible" % (N, dt, n, t_setup, el, _cpu_model(), os.cpu_count())}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def traffic_from_profile(kernel, N):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if one exists."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(str(N), {}).get(kernel, {}).get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--dt", type=float, default=0.004)
+    ap.add_argument("--no-512", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one device-to-device copy of this many MiB first (PMC counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each solve runs on one device; "
+              "ensembles shard by solve). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no GPU visible"
+    torch.cuda.set_device(0)
+    if a.copy_probe_mib:
+        n = a.copy_probe_mib * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+    main_r = measure(a.N, a.steps, a.warmup, a.dt, "headline")
+    extra = None
+    if not a.no_512 and a.N != 512:
+        extra = measure(512, max(4, a.steps // 4), 1, 0.002, "512^3")
+    cpu = None if a.no_cpu else cpu_baseline(min(a.N, 256), a.dt)
+    dom = main_r["dominant"]
+    k = main_r["kernels"][dom]
+    traffic = traffic_from_profile(dom, a.N)
+    out = {
+        "metric": "FP64 RK4 steps/s at %d^3 (TGV, %% of HBM roofline in roofline)" % a.N,
+        "value": main_r["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": main_r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Taylor-Green vortex %d^3, nu=1/1600, dt=%g fixed, FP64, 2/3 dealiased, RK4" % (a.N, a.dt),
+                   "N": a.N, "nu": 1 / 1600, "dt": a.dt, "ic": "tgv"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": k["alg_bytes"], "avg_launch_us": k["avg_us"]},
+        "step_roofline_frac": main_r["step_roofline_frac"],
+        "b_alg_per_step_GB": b_alg_step(a.N) / 1e9,
+        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1), "launches": v["launches"]}
+                    for n, v in main_r["kernels"].items()},
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        out["n512"] = {"workload": "Taylor-Green 512^3 FP64, dt=0.002 fixed", "steps_per_s": extra["steps_per_s"],
+                       "ms_per_step": extra["ms_per_step"], "step_roofline_frac": extra["step_roofline_frac"],
+                       "dominant": extra["dominant"],
+                       "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
+                       "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
+                                   for n, v in extra["kernels"].items()}}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

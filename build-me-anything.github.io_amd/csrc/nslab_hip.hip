@@ -269,7 +269,48 @@ struct nsl_solver {
   double *phys = nullptr;   // [15][N^3]
   double *dpart = nullptr;  // diagnose partials
   int diag_valid = 0;
+  // optional per-kernel timing with HIP events on the solver's stream (nsl_profile)
+  int prof = 0;
+  std::vector<hipEvent_t> evs;
+  std::vector<int> evcls;
+  size_t evn = 0;
+  double prof_ms[NSL_PROF_CLASSES] = {0};
+  int64_t prof_cnt[NSL_PROF_CLASSES] = {0};
 };
+
+// Event pair around one launch when profiling is on.
+static int prof_begin(nsl_solver *s, int cls) {
+  if (!s->prof) return NSL_OK;
+  if (s->evn + 2 > s->evs.size()) {
+    for (int i = 0; i < 2; i++) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      s->evs.push_back(e);
+    }
+    s->evcls.resize(s->evs.size() / 2);
+  }
+  s->evcls[s->evn / 2] = cls;
+  HIP_TRY(hipEventRecord(s->evs[s->evn], s->st));
+  return NSL_OK;
+}
+static int prof_end(nsl_solver *s) {
+  if (!s->prof) return NSL_OK;
+  HIP_TRY(hipEventRecord(s->evs[s->evn + 1], s->st));
+  s->evn += 2;
+  return NSL_OK;
+}
+// After a stream synchronisation: fold the recorded pairs into the per-class totals.
+static int prof_collect(nsl_solver *s) {
+  for (size_t i = 0; i + 1 < s->evn; i += 2) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, s->evs[i], s->evs[i + 1]));
+    const int c = s->evcls[i / 2];
+    s->prof_ms[c] += ms;
+    s->prof_cnt[c]++;
+  }
+  s->evn = 0;
+  return NSL_OK;
+}
 
 template <typename T>
 static int dalloc(nsl_solver *s, T **p, size_t count) {
@@ -284,10 +325,11 @@ static int dalloc(nsl_solver *s, T **p, size_t count) {
     if (r_ != NSL_OK) return r_; \
   } while (0)
 
+static int prof_collect(nsl_solver *s);
 static int pull_ctl(nsl_solver *s) {
   HIP_TRY(hipMemcpyAsync(s->hctl, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
-  return NSL_OK;
+  return prof_collect(s);
 }
 static int push_ctl(nsl_solver *s) {
   HIP_TRY(hipMemcpyAsync(s->ctl, s->hctl, sizeof(Ctl), hipMemcpyHostToDevice, s->st));
@@ -301,12 +343,13 @@ static void diag_from(const double *d, nsl_rhsdiag &o) {
 // One rhs of U with the fused stage-`stage` update; fmode selects where k_finalize books the sums.
 static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecDev *rec) {
   const Xform &x = s->x;
-  TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl));
-  TRY(yinv(x, s->X1, s->X2, 6, s->ctl));
-  TRY(xnl(x, s->X2, s->ctl));
-  TRY(yfwd(x, s->X2, s->X1, 3, s->ctl));
-  TRY(zfwd(x, s->X1, s->NL, 3, s->ctl));
+  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_ZFWD)); TRY(zfwd(x, s->X1, s->NL, 3, s->ctl)); TRY(prof_end(s));
   dim3 g((unsigned)s->nblk), b(NSL_NT);
+  TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
   switch (stage) {
     case 1: hipLaunchKernelGGL(k_update<1>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
     case 2: hipLaunchKernelGGL(k_update<2>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
@@ -314,8 +357,11 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
     default: hipLaunchKernelGGL(k_update<4>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
   }
   LAUNCH_CHECK();
+  TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_FINALIZE));
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSL_NT), 0, s->st, s->partials, s->nblk, fmode, rec, s->ctl);
   LAUNCH_CHECK();
+  TRY(prof_end(s));
   return NSL_OK;
 }
 
@@ -478,6 +524,7 @@ static void solver_free(nsl_solver *s) {
   if (s->hrecs) (void)hipHostFree(s->hrecs);
   if (s->hctl) (void)hipHostFree(s->hctl);
   xform_free(s->x);
+  for (auto e : s->evs) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
 }
@@ -736,6 +783,21 @@ int nsl_set_state(nsl_solver *s, const double *const planes[6]) {
 }
 
 int64_t nsl_device_bytes(const nsl_solver *s) { return s ? s->bytes : 0; }
+
+int nsl_profile(nsl_solver *s, int32_t on) {
+  if (!s) return fail(NSL_ERR_ARG, "null solver");
+  TRY(pull_ctl(s));
+  s->prof = on ? 1 : 0;
+  for (int i = 0; i < NSL_PROF_CLASSES; i++) { s->prof_ms[i] = 0; s->prof_cnt[i] = 0; }
+  return NSL_OK;
+}
+
+int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count) {
+  if (!s || !ms || !count) return fail(NSL_ERR_ARG, "null argument");
+  TRY(pull_ctl(s));
+  for (int i = 0; i < NSL_PROF_CLASSES; i++) { ms[i] = s->prof_ms[i]; count[i] = s->prof_cnt[i]; }
+  return NSL_OK;
+}
 
 int nsl_fft1(int32_t N, int32_t sign, int64_t lines, double *re, double *im) {
   if (!re || !im) return fail(NSL_ERR_ARG, "null argument");

@@ -1,0 +1,403 @@
+/*
+ * nslab_napi.node — N-API binding of the C ABI in include/nslab_hip.h for the Node host module js/nslab.js.
+ * Built with gcc against node_api.h only (no node-gyp); links libnslab_hip.so from the same directory.
+ * Every nonzero status from the library becomes a JS Error carrying nsl_last_error()'s text.
+ */
+#include <node_api.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "nslab_hip.h"
+
+#define CALL(env, expr)                                                     \
+  do {                                                                      \
+    if ((expr) != napi_ok) {                                                \
+      napi_throw_error((env), NULL, "N-API call failed: " #expr);           \
+      return NULL;                                                          \
+    }                                                                       \
+  } while (0)
+
+#define ARGS(n)                                                             \
+  size_t argc = (n);                                                        \
+  napi_value argv[(n) > 0 ? (n) : 1];                                       \
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL))
+
+static napi_value throw_status(napi_env env, int rc) {
+  const char *m = nsl_last_error();
+  napi_throw_error(env, NULL, (m && m[0]) ? m : "nslab-hip error");
+  (void)rc;
+  return NULL;
+}
+
+static int get_f64(napi_env env, napi_value v, double *out) { return napi_get_value_double(env, v, out) == napi_ok; }
+static int get_i64(napi_env env, napi_value v, int64_t *out) { return napi_get_value_int64(env, v, out) == napi_ok; }
+
+/* Float64Array -> pointer and element count */
+static int get_f64arr(napi_env env, napi_value v, double **p, size_t *n) {
+  napi_typedarray_type t;
+  size_t len, off;
+  void *data;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, v, &t, &len, &data, &ab, &off) != napi_ok || t != napi_float64_array) return 0;
+  *p = (double *)data;
+  *n = len;
+  return 1;
+}
+
+static napi_value new_f64arr(napi_env env, size_t n, double **data) {
+  napi_value ab, ta;
+  void *p;
+  if (napi_create_arraybuffer(env, n * sizeof(double), &p, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta) != napi_ok) return NULL;
+  *data = (double *)p;
+  return ta;
+}
+
+static void finalize_solver(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  nsl_destroy((nsl_solver *)data);
+}
+
+static nsl_solver *get_solver(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  return (nsl_solver *)p;
+}
+
+#define SOLVER(idx)                                                         \
+  nsl_solver *s = get_solver(env, argv[idx]);                               \
+  if (!s) { napi_throw_type_error(env, NULL, "not a solver handle"); return NULL; }
+
+/* create(N, ic, nu, cfl, dt, icp0..icp3) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  ARGS(9);
+  nsl_opts o;
+  memset(&o, 0, sizeof o);
+  int64_t N = 0, ic = 0;
+  if (argc < 9 || !get_i64(env, argv[0], &N) || !get_i64(env, argv[1], &ic) || !get_f64(env, argv[2], &o.nu) ||
+      !get_f64(env, argv[3], &o.cfl) || !get_f64(env, argv[4], &o.dt)) {
+    napi_throw_type_error(env, NULL, "create(N, ic, nu, cfl, dt, p0, p1, p2, p3)");
+    return NULL;
+  }
+  for (int i = 0; i < 4; i++) if (!get_f64(env, argv[5 + i], &o.icp[i])) o.icp[i] = 0;
+  o.N = (int32_t)N;
+  o.ic = (int32_t)ic;
+  nsl_solver *s = NULL;
+  int rc = nsl_create(&o, &s);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value ext;
+  CALL(env, napi_create_external(env, s, finalize_solver, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_set_ic(napi_env env, napi_callback_info info) {
+  ARGS(6);
+  SOLVER(0);
+  int64_t ic = 0;
+  double p[4] = {0, 0, 0, 0};
+  get_i64(env, argv[1], &ic);
+  for (int i = 0; i < 4; i++) get_f64(env, argv[2 + i], &p[i]);
+  int rc = nsl_set_ic(s, (int32_t)ic, p);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+/* run(h, n, hasTEnd, tEnd) -> Float64Array(nDone * 21): t dt step ebal zbal last[8] d0[8] */
+static napi_value js_run(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  SOLVER(0);
+  int64_t n = 0, has = 0;
+  double tend = 0;
+  get_i64(env, argv[1], &n);
+  get_i64(env, argv[2], &has);
+  get_f64(env, argv[3], &tend);
+  if (n < 0) n = 0;
+  nsl_steprec *recs = (nsl_steprec *)calloc(n > 0 ? (size_t)n : 1, sizeof(nsl_steprec));
+  if (!recs) { napi_throw_error(env, NULL, "out of host memory"); return NULL; }
+  int64_t done = 0;
+  int rc = nsl_run(s, n, (int32_t)has, tend, recs, &done);
+  if (rc < 0) { free(recs); return throw_status(env, rc); }
+  double *d;
+  napi_value out = new_f64arr(env, (size_t)done * 21, &d);
+  for (int64_t i = 0; i < done; i++) {
+    const nsl_steprec *r = &recs[i];
+    double *o = d + i * 21;
+    o[0] = r->t; o[1] = r->dt; o[2] = (double)r->step; o[3] = r->ebal; o[4] = r->zbal;
+    memcpy(o + 5, &r->last, 8 * sizeof(double));
+    memcpy(o + 13, &r->d0, 8 * sizeof(double));
+  }
+  free(recs);
+  return out;
+}
+
+static napi_value js_last(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  nsl_rhsdiag d;
+  int rc = nsl_last(s, &d);
+  if (rc < 0) return throw_status(env, rc);
+  double *p;
+  napi_value out = new_f64arr(env, 8, &p);
+  memcpy(p, &d, sizeof d);
+  return out;
+}
+
+static napi_value js_set_clock(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  SOLVER(0);
+  double t = 0, E0 = 0;
+  int64_t step = 0;
+  get_f64(env, argv[1], &t);
+  get_i64(env, argv[2], &step);
+  get_f64(env, argv[3], &E0);
+  int rc = nsl_set_clock(s, t, step, E0);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_get_clock(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  double t, dt;
+  int64_t step;
+  int rc = nsl_get_clock(s, &t, &step, &dt);
+  if (rc < 0) return throw_status(env, rc);
+  double *p;
+  napi_value out = new_f64arr(env, 3, &p);
+  p[0] = t; p[1] = (double)step; p[2] = dt;
+  return out;
+}
+
+static napi_value js_norms(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  double *p;
+  napi_value out = new_f64arr(env, 3, &p);
+  int rc = nsl_norms(s, p);
+  if (rc < 0) return throw_status(env, rc);
+  return out;
+}
+
+/* getState(h, [6 Float64Array]) / setState(h, [6 Float64Array]) */
+static int planes_of(napi_env env, napi_value arr, double *planes[6], size_t want) {
+  for (uint32_t i = 0; i < 6; i++) {
+    napi_value e;
+    size_t n;
+    if (napi_get_element(env, arr, i, &e) != napi_ok || !get_f64arr(env, e, &planes[i], &n) || n < want) return 0;
+  }
+  return 1;
+}
+
+static napi_value js_get_state(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SOLVER(0);
+  int64_t NS = 0;
+  get_i64(env, argv[2], &NS);
+  double *pl[6];
+  if (!planes_of(env, argv[1], pl, (size_t)NS)) { napi_throw_type_error(env, NULL, "state needs 6 Float64Array of NS"); return NULL; }
+  int rc = nsl_get_state(s, pl);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_set_state(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SOLVER(0);
+  int64_t NS = 0;
+  get_i64(env, argv[2], &NS);
+  double *pl[6];
+  if (!planes_of(env, argv[1], pl, (size_t)NS)) { napi_throw_type_error(env, NULL, "state needs 6 Float64Array of NS"); return NULL; }
+  int rc = nsl_set_state(s, (const double *const *)pl);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_diagnose(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  SOLVER(0);
+  int64_t interp = 1;
+  get_i64(env, argv[1], &interp);
+  int nb = nsl_spectrum_len(s);
+  if (nb < 0) return throw_status(env, nb);
+  double *p;
+  napi_value out = new_f64arr(env, NSL_DIAG_LEN + (size_t)nb, &p);
+  int rc = nsl_diagnose(s, (int32_t)interp, p);
+  if (rc < 0) return throw_status(env, rc);
+  return out;
+}
+
+static napi_value js_spectrum(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  int nb = nsl_spectrum_len(s);
+  if (nb < 0) return throw_status(env, nb);
+  double *p;
+  napi_value out = new_f64arr(env, (size_t)nb, &p);
+  int rc = nsl_spectrum(s, p);
+  if (rc < 0) return throw_status(env, rc);
+  return out;
+}
+
+static napi_value js_interp_max(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SOLVER(0);
+  int64_t top = 24, iters = 12;
+  get_i64(env, argv[1], &top);
+  get_i64(env, argv[2], &iters);
+  double *p;
+  napi_value out = new_f64arr(env, 5, &p);
+  int rc = nsl_interp_max(s, (int32_t)top, (int32_t)iters, p);
+  if (rc < 0) return throw_status(env, rc);
+  return out;
+}
+
+/* field(h, which, Float64Array(N^3)) */
+static napi_value js_field(napi_env env, napi_callback_info info) {
+  ARGS(3);
+  SOLVER(0);
+  int64_t which = 0;
+  get_i64(env, argv[1], &which);
+  double *p;
+  size_t n;
+  if (!get_f64arr(env, argv[2], &p, &n)) { napi_throw_type_error(env, NULL, "field needs a Float64Array"); return NULL; }
+  int rc = nsl_field(s, (int32_t)which, p);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_exact_error(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  double e[2];
+  int rc = nsl_exact_error(s, e);
+  if (rc < 0) return throw_status(env, rc);
+  if (rc == 0) { napi_value nul; CALL(env, napi_get_null(env, &nul)); return nul; }
+  double *p;
+  napi_value out = new_f64arr(env, 2, &p);
+  p[0] = e[0]; p[1] = e[1];
+  return out;
+}
+
+static napi_value js_device_bytes(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  napi_value v;
+  CALL(env, napi_create_double(env, (double)nsl_device_bytes(s), &v));
+  return v;
+}
+
+/* fft1(N, sign, re, im): in place on Float64Arrays holding lines of N */
+static napi_value js_fft1(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  int64_t N = 0, sign = 0;
+  double *re, *im;
+  size_t nr, ni;
+  get_i64(env, argv[0], &N);
+  get_i64(env, argv[1], &sign);
+  if (!get_f64arr(env, argv[2], &re, &nr) || !get_f64arr(env, argv[3], &im, &ni) || nr != ni || N <= 0 || nr % (size_t)N) {
+    napi_throw_type_error(env, NULL, "fft1(N, sign, Float64Array re, Float64Array im)");
+    return NULL;
+  }
+  int rc = nsl_fft1((int32_t)N, (int32_t)sign, (int64_t)(nr / (size_t)N), re, im);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_spectral_forward(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  int64_t N = 0;
+  double *f, *re, *im;
+  size_t n1, n2, n3;
+  get_i64(env, argv[0], &N);
+  if (!get_f64arr(env, argv[1], &f, &n1) || !get_f64arr(env, argv[2], &re, &n2) || !get_f64arr(env, argv[3], &im, &n3)) {
+    napi_throw_type_error(env, NULL, "spectralForward(N, f, re, im)");
+    return NULL;
+  }
+  int rc = nsl_spectral_forward((int32_t)N, f, re, im);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_spectral_inverse(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  int64_t N = 0;
+  double *f, *re, *im;
+  size_t n1, n2, n3;
+  get_i64(env, argv[0], &N);
+  if (!get_f64arr(env, argv[1], &re, &n1) || !get_f64arr(env, argv[2], &im, &n2) || !get_f64arr(env, argv[3], &f, &n3)) {
+    napi_throw_type_error(env, NULL, "spectralInverse(N, re, im, f)");
+    return NULL;
+  }
+  int rc = nsl_spectral_inverse((int32_t)N, re, im, f);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_factorize(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  int64_t N = 0;
+  get_i64(env, argv[0], &N);
+  int32_t f[64];
+  int k = nsl_factorize((int32_t)N, f, 64);
+  if (k < 0) return throw_status(env, k);
+  napi_value arr;
+  CALL(env, napi_create_array_with_length(env, (size_t)k, &arr));
+  for (int i = 0; i < k; i++) {
+    napi_value v;
+    CALL(env, napi_create_int32(env, f[i], &v));
+    CALL(env, napi_set_element(env, arr, (uint32_t)i, v));
+  }
+  return arr;
+}
+
+static napi_value js_profile(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  SOLVER(0);
+  int64_t on = 0;
+  get_i64(env, argv[1], &on);
+  int rc = nsl_profile(s, (int32_t)on);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_profile_read(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  double ms[NSL_PROF_CLASSES];
+  int64_t cnt[NSL_PROF_CLASSES];
+  int rc = nsl_profile_read(s, ms, cnt);
+  if (rc < 0) return throw_status(env, rc);
+  double *p;
+  napi_value out = new_f64arr(env, 2 * NSL_PROF_CLASSES, &p);
+  for (int i = 0; i < NSL_PROF_CLASSES; i++) { p[i] = ms[i]; p[NSL_PROF_CLASSES + i] = (double)cnt[i]; }
+  return out;
+}
+
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value v;
+  CALL(env, napi_create_string_utf8(env, nsl_version(), NAPI_AUTO_LENGTH, &v));
+  return v;
+}
+
+static napi_value Init(napi_env env, napi_value exports) {
+  static const struct { const char *name; napi_callback cb; } fns[] = {
+    {"create", js_create}, {"setIC", js_set_ic}, {"run", js_run}, {"last", js_last},
+    {"setClock", js_set_clock}, {"getClock", js_get_clock}, {"norms", js_norms},
+    {"getState", js_get_state}, {"setState", js_set_state}, {"diagnose", js_diagnose},
+    {"spectrum", js_spectrum}, {"interpMax", js_interp_max}, {"field", js_field},
+    {"exactError", js_exact_error}, {"deviceBytes", js_device_bytes}, {"fft1", js_fft1},
+    {"spectralForward", js_spectral_forward}, {"spectralInverse", js_spectral_inverse},
+    {"factorize", js_factorize}, {"profile", js_profile}, {"profileRead", js_profile_read},
+    {"version", js_version},
+  };
+  for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
+    napi_value f;
+    CALL(env, napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].cb, NULL, &f));
+    CALL(env, napi_set_named_property(env, exports, fns[i].name, f));
+  }
+  return exports;
+}
+
+NAPI_MODULE(nslab_napi, Init)

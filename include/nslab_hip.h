@@ -114,6 +114,22 @@ int nsl_exact_error(nsl_solver *s, double out[2]);
 /* Bytes of device memory the solver holds (admission accounting). */
 int64_t nsl_device_bytes(const nsl_solver *s);
 
+/* Per-kernel timing (no reference counterpart: the measurement hook the benchmark uses). With profiling on, every
+ * launch of a step is bracketed by HIP events on the solver's stream; nsl_profile_read returns the summed
+ * milliseconds and launch counts per kernel class since nsl_profile(s, 1) reset them. */
+#define NSL_PROF_ZINV 0      /* P1: z-inverse + curl */
+#define NSL_PROF_YINV 1      /* P2: y-inverse */
+#define NSL_PROF_XNL 2       /* P3: x C2R, u x omega, x R2C */
+#define NSL_PROF_YFWD 3      /* P4: y-forward */
+#define NSL_PROF_ZFWD 4      /* P5: z-forward */
+#define NSL_PROF_UPDATE1 5   /* projection + RK4 stage 1 */
+#define NSL_PROF_UPDATE23 6  /* projection + RK4 stages 2, 3 */
+#define NSL_PROF_UPDATE4 7   /* projection + RK4 stage 4 + norms */
+#define NSL_PROF_FINALIZE 8  /* fixed-order sum of the partials, step bookkeeping */
+#define NSL_PROF_CLASSES 9
+int nsl_profile(nsl_solver *s, int32_t on);
+int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count);
+
 /* NS.fft1 / NS.Spectral (nslab.js:46-155) on the GPU: `lines` independent length-N lines (split re/im, line-major)
  * transformed in place, sign -1 forward, +1 inverse, unnormalised. */
 int nsl_fft1(int32_t N, int32_t sign, int64_t lines, double *re, double *im);

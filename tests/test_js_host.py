@@ -1,0 +1,39 @@
+"""The Node host module (build-me-anything.github.io_amd/js/nslab.js) over the N-API addon."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SCRIPT = os.path.join(ROOT, "tests", "js", "validate_nslab.js")
+MODULE = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "nslab.js")
+
+node = shutil.which("node")
+needs_node = pytest.mark.skipif(node is None, reason="node is not installed")
+
+
+@needs_node
+def test_module_loads_and_exports_reference_surface():
+    """No GPU needed: the addon loads, the export list is the reference's, factorize throws its message."""
+    js = ("const NS = require(%r);"
+          "const keys = ['VERSION','IC_INFO','createSolver','studySummary','Spectral','fftPlan','fft1','factorize'];"
+          "for (const k of keys) if (!(k in NS)) throw new Error('missing ' + k);"
+          "if (JSON.stringify(NS.factorize(96)) !== '[4,4,2,3]') throw new Error('factorize');"
+          "let m = ''; try { NS.factorize(7); } catch (e) { m = e.message; }"
+          "if (m !== 'grid size must be 2^a·3^b ≥ 4 (got 7)') throw new Error('message ' + m);"
+          "const s = NS.studySummary([{label:'a',N:16,peak:{omMax:1},epsPeak:{v:1,t:1},Eend:1},"
+          "{label:'b',N:32,peak:{omMax:1.001},epsPeak:{v:1.001,t:1},Eend:1}], 'grid');"
+          "if (s.verdict !== 'PASS') throw new Error('verdict');"
+          "console.log('OK');") % MODULE
+    r = subprocess.run([node, "-e", js], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "OK" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
+@pytest.mark.gpu
+@pytest.mark.parametrize("group", ["fft", "exact", "tgv", "misc", "config1"])
+def test_validate_nslab_on_gpu(group):
+    r = subprocess.run([node, SCRIPT, group], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
