@@ -20,7 +20,8 @@ namespace nsl {
 // an untruncated transform uses hx = N/2+1, by = N, hy = N/2.
 struct Geo {
   int N, NH, hx, by, hy;
-  long long D;   // modes per component in compact storage = by*by*hx
+  int hp;        // row pitch along kx (hx rounded up to 8 complex = 128 bytes for the solver state and pencils)
+  long long D;   // slots per component in compact storage = by*by*hp
 };
 
 struct DPlan {
@@ -34,6 +35,13 @@ __device__ __forceinline__ int full_of(int c, const Geo &g) { return c < g.hy ? 
 __device__ __forceinline__ int compact_of(int f, const Geo &g) {
   return f < g.hy ? f : (f >= g.N - g.by + g.hy ? f - (g.N - g.by) : -1);
 }
+// The dispatcher deals workgroups to the 8 XCDs round-robin, and each XCD has its own L2. This maps block ids so
+// each XCD works on one contiguous range of logical tiles: tiles that share inputs then share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int per = nblocks / 8, rem = nblocks % 8, x = bid % 8, l = bid / 8;
+  return x * per + min(x, rem) + l;
+}
+
 // signed wavenumber of full index f (nslab.js:180: j <= N/2 ? j : j - N)
 __device__ __forceinline__ double kwv(int f, int N) { return (double)(f <= (N >> 1) ? f : f - N); }
 
@@ -152,6 +160,21 @@ __device__ __forceinline__ void block_atomic_max2(double a, double b, unsigned l
     for (int w = 1; w < NSL_NT / 64; w++) { ma = fmax(ma, scratch[2 * w]); mb = fmax(mb, scratch[2 * w + 1]); }
     atomicMax(da, (unsigned long long)__double_as_longlong(ma));
     atomicMax(db, (unsigned long long)__double_as_longlong(mb));
+  }
+}
+
+// Max of two non-negative doubles over the workgroup, stored (not atomically combined) at dst[0], dst[1]: every
+// workgroup owns its own slot, so no two workgroups contend for one address.
+__device__ __forceinline__ void block_max2_store(double a, double b, double *dst, double *scratch) {
+  a = wave_max(a); b = wave_max(b);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) { scratch[2 * wid] = a; scratch[2 * wid + 1] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ma = scratch[0], mb = scratch[1];
+    for (int w = 1; w < NSL_NT / 64; w++) { ma = fmax(ma, scratch[2 * w]); mb = fmax(mb, scratch[2 * w + 1]); }
+    dst[0] = ma; dst[1] = mb;
   }
 }
 

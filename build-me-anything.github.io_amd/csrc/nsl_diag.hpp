@@ -147,9 +147,9 @@ __global__ __launch_bounds__(NSL_NT) void k_spectrum(const double2 *__restrict__
     double e = 0;
     if (m < hi) {
       double kx, ky, kz;
-      mode_k(m, g, kx, ky, kz);
+      const bool real_mode = mode_k(m, g, kx, ky, kz);
       const int k = (int)floor(sqrt(kx * kx + ky * ky + kz * kz) + 0.5);
-      if (k < nb) {
+      if (real_mode && k < nb) {
         const double2 a = U[m], b = U[D + m], c = U[2 * D + m];
         e = a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y + c.x * c.x + c.y * c.y;
         const double hw = (kx == 0 || 2 * kx == g.N) ? 1.0 : 2.0;
@@ -220,7 +220,7 @@ __global__ void k_curl(const double2 *__restrict__ U, double2 *__restrict__ W, c
   const long long D = g.D;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < D; m += (long long)gridDim.x * blockDim.x) {
     double kx, ky, kz;
-    mode_k(m, g, kx, ky, kz);
+    if (!mode_k(m, g, kx, ky, kz)) continue;
     const double2 a = U[m], b = U[D + m], c = U[2 * D + m];
     double pr = ky * c.x - kz * b.x, pi = ky * c.y - kz * b.y;
     W[m] = make_double2(-pi, pr);
@@ -237,7 +237,7 @@ __global__ void k_tens_a(const double2 *__restrict__ IW, const Geo g, int c, dou
   const int rows = g.by * g.by;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
-  const double2 *w = IW + (size_t)c * g.D + (size_t)r * g.hx;
+  const double2 *w = IW + (size_t)c * g.D + (size_t)r * g.hp;
   double s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
   for (int i = 0; i < g.hx; i++) {
     const double kxx = i, ph = kxx * x0, cr = cos(ph), ci = sin(ph), hw = (i == 0 || 2 * i == g.N) ? 1.0 : 2.0;

@@ -1,0 +1,275 @@
+// Size-specialised FFT engine for the pencil passes. The line length N, the lines per workgroup T and the radix
+// schedule are compile-time constants, so every index is a shift or a multiply by a constant and every loop unrolls.
+// Each stage is a radix-R Stockham decimation-in-frequency step (the same recurrence as nslab.js:46-92, with larger
+// radices: an R-point DFT in registers, then the twiddle W_N^(j*p*s) on leg j). The first stage loads its inputs
+// straight from global memory and the last stage stores straight to global memory, so LDS is touched only between
+// stages: once for N = 16*16, twice for 8*8*8.
+#pragma once
+#include "nsl_device.hpp"
+
+namespace nsl {
+namespace fast {
+
+// The transform arithmetic may fuse a product into the following add (one rounding instead of two); everything
+// outside the FFT engines keeps the reference's separate roundings (-ffp-contract=off for the library).
+#ifndef NSL_FFT_CONTRACT
+#define NSL_FFT_CONTRACT _Pragma("clang fp contract(fast)")
+#endif
+
+template <int SIGN>
+__device__ __forceinline__ double2 mul_tw(double2 a, double2 w) {   // a * (w.x + i SIGN w.y)
+  NSL_FFT_CONTRACT
+  const double c = w.x, d = SIGN * w.y;
+  return make_double2(a.x * c - a.y * d, a.x * d + a.y * c);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by SIGN * i
+template <int SIGN>
+__device__ __forceinline__ double2 mul_si(double2 a) { return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
+
+// In-register DFTs: y_j = sum_k x_k w^(jk), w = exp(SIGN 2 pi i / R), natural order in and out.
+template <int R, int SIGN> struct Dft;
+
+template <int SIGN> struct Dft<2, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+  }
+};
+
+template <int SIGN> struct Dft<3, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    NSL_FFT_CONTRACT
+    const double S3 = -SIGN * 0.8660254037844386;
+    const double2 a0 = v[0], a1 = v[1], a2 = v[2];
+    const double sr = a1.x + a2.x, si = a1.y + a2.y, ur = a0.x - 0.5 * sr, ui = a0.y - 0.5 * si;
+    const double vr = S3 * (a1.x - a2.x), vi = S3 * (a1.y - a2.y);
+    v[0] = make_double2(a0.x + sr, a0.y + si);
+    v[1] = make_double2(ur + vi, ui - vr);
+    v[2] = make_double2(ur - vi, ui + vr);
+  }
+};
+
+template <int SIGN> struct Dft<4, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+    const double2 st3 = mul_si<SIGN>(t3);
+    v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+    v[1] = cadd(t1, st3); v[3] = csub(t1, st3);
+  }
+};
+
+template <int SIGN> struct Dft<8, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    NSL_FFT_CONTRACT
+    const double H = 0.7071067811865476;
+    double2 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { a[k] = cadd(v[k], v[k + 4]); b[k] = csub(v[k], v[k + 4]); }
+    // b_k *= w8^k, w8 = exp(SIGN i pi/4)
+    b[1] = make_double2(H * (b[1].x - SIGN * b[1].y), H * (b[1].y + SIGN * b[1].x));
+    b[2] = mul_si<SIGN>(b[2]);
+    b[3] = make_double2(H * (-b[3].x - SIGN * b[3].y), H * (-b[3].y + SIGN * b[3].x));
+    Dft<4, SIGN>::run(a);
+    Dft<4, SIGN>::run(b);
+#pragma unroll
+    for (int m = 0; m < 4; m++) { v[2 * m] = a[m]; v[2 * m + 1] = b[m]; }
+  }
+};
+
+template <int SIGN> struct Dft<16, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    // n = 4 n1 + n2, j = j1 + 4 j2: DFT4 over n1, twiddle w16^(j1 n2), DFT4 over n2
+    const double C1 = 0.9238795325112867, S1 = 0.3826834323650898, H = 0.7071067811865476;
+    double2 A[4][4];   // A[n2][j1]
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+      double2 c[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<4, SIGN>::run(c);
+#pragma unroll
+      for (int j1 = 0; j1 < 4; j1++) A[n2][j1] = c[j1];
+    }
+    // twiddles w16^(j1 n2) = exp(SIGN i 2 pi j1 n2 / 16) for j1, n2 in 1..3
+    A[1][1] = mul_tw<SIGN>(A[1][1], make_double2(C1, S1));
+    A[1][2] = mul_tw<SIGN>(A[1][2], make_double2(H, H));
+    A[1][3] = mul_tw<SIGN>(A[1][3], make_double2(S1, C1));
+    A[2][1] = mul_tw<SIGN>(A[2][1], make_double2(H, H));
+    A[2][2] = mul_si<SIGN>(A[2][2]);
+    A[2][3] = mul_tw<SIGN>(A[2][3], make_double2(-H, H));
+    A[3][1] = mul_tw<SIGN>(A[3][1], make_double2(S1, C1));
+    A[3][2] = mul_tw<SIGN>(A[3][2], make_double2(-H, H));
+    A[3][3] = mul_tw<SIGN>(A[3][3], make_double2(-C1, -S1));
+#pragma unroll
+    for (int j1 = 0; j1 < 4; j1++) {
+      double2 c[4] = {A[0][j1], A[1][j1], A[2][j1], A[3][j1]};
+      Dft<4, SIGN>::run(c);
+#pragma unroll
+      for (int j2 = 0; j2 < 4; j2++) v[j1 + 4 * j2] = c[j2];
+    }
+  }
+};
+
+// Radix schedules (product = N). Larger radices first keep register use flat.
+template <int N> struct Sched;
+template <int... Rs> struct RList {};
+template <> struct Sched<16> { using L = RList<16>; };
+template <> struct Sched<24> { using L = RList<8, 3>; };
+template <> struct Sched<32> { using L = RList<8, 4>; };
+template <> struct Sched<48> { using L = RList<16, 3>; };
+template <> struct Sched<64> { using L = RList<8, 8>; };
+template <> struct Sched<96> { using L = RList<8, 4, 3>; };
+template <> struct Sched<128> { using L = RList<16, 8>; };
+template <> struct Sched<192> { using L = RList<16, 4, 3>; };
+template <> struct Sched<256> { using L = RList<16, 16>; };
+template <> struct Sched<384> { using L = RList<16, 8, 3>; };
+template <> struct Sched<512> { using L = RList<8, 8, 8>; };
+template <> struct Sched<768> { using L = RList<16, 16, 3>; };
+template <> struct Sched<1024> { using L = RList<16, 8, 8>; };
+
+// LDS layouts. Interleaved (pencil passes, T a multiple of 8): element n of line t at n*T + t, threads walk t
+// fastest so global loads and stores coalesce across lines. LineMajor (x pass, few lines): line t holds its N
+// elements contiguously with one pad slot after every 8, and threads walk the butterfly index fastest, so the
+// radix-stride writes of the first stage spread over the LDS banks.
+template <int N, int T> struct Interleaved {
+  static constexpr int SIZE = N * T;
+  static __device__ __forceinline__ int at(int n, int t) { return n * T + t; }
+  template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { t = idx % T; bb = idx / T; }
+};
+template <int N, int T> struct LineMajor {
+  static constexpr int STRIDE = N + N / 8, SIZE = T * STRIDE;
+  static __device__ __forceinline__ int at(int n, int t) { return t * STRIDE + n + (n >> 3); }
+  template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { bb = idx % NB; t = idx / NB; }
+};
+
+// One stage. S = product of the radices before it. Input either from `load(n, t)` (FIRST) or LDS; output to
+// `store(n, t, v)` (LAST) or LDS, at the layout LY's addresses.
+template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, class LY, class LD,
+          class ST>
+__device__ __forceinline__ void stage(double2 *__restrict__ lds, const double2 *__restrict__ tw, LD &load, ST &store) {
+  constexpr int NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
+  double2 v[K][R];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (TOT % NT == 0 || idx < TOT) {
+      int t, bb;
+      LY::template split<NB>(idx, t, bb);
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int n = bb + j * NB;
+        v[k][j] = FIRST ? load(n, t) : lds[LY::at(n, t)];
+      }
+    }
+  }
+  if (!FIRST || SYNC_FIRST) __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (TOT % NT == 0 || idx < TOT) {
+      int t, bb;
+      LY::template split<NB>(idx, t, bb);
+      const int p = bb / S, q = bb % S;
+      Dft<R, SIGN>::run(v[k]);
+      if (S * R != N) {
+#pragma unroll
+        for (int j = 1; j < R; j++) v[k][j] = mul_tw<SIGN>(v[k][j], tw[j * p * S]);
+      }
+      const int o0 = q + S * R * p;
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int n = o0 + j * S;
+        if (LAST) store(n, t, v[k][j]);
+        else lds[LY::at(n, t)] = v[k][j];
+      }
+    }
+  }
+  if (!LAST) __syncthreads();
+}
+
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, class LD, class ST, int R0,
+          int... Rs>
+__device__ __forceinline__ void run_stages(double2 *lds, const double2 *tw, LD &load, ST &store, RList<R0, Rs...>) {
+  constexpr bool LAST = sizeof...(Rs) == 0;
+  stage<N, T, NT, R0, S, FIRST, LAST, SIGN, SYNC_FIRST, LY>(lds, tw, load, store);
+  if constexpr (!LAST) run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, S * R0, false>(lds, tw, load, store, RList<Rs...>{});
+}
+
+// Full transform of T lines of length N. tw: the N roots of unity (cos, sin)(2 pi k / N). SYNC_FIRST: the first
+// stage's loads read LDS that the stage then overwrites (in-place use), so a barrier separates them.
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleaved<N, T>, class LD, class ST>
+__device__ __forceinline__ void fft(double2 *lds, const double2 *tw, LD &load, ST &store) {
+  run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, typename Sched<N>::L{});
+}
+
+// ------------------------------------------------------------------------------------------------ split form
+// The first stage's inputs held in registers, so a persistent workgroup can issue the loads of its next tile before
+// it transforms the current one (the loads of tile i+1 are in flight during the LDS stages of tile i).
+template <class L> struct FirstR;
+template <int R0, int... Rs> struct FirstR<RList<R0, Rs...>> {
+  static constexpr int R = R0;
+  using Rest = RList<Rs...>;
+  static constexpr bool single = sizeof...(Rs) == 0;
+};
+
+template <int N, int T, int NT> struct Regs {
+  using F = FirstR<typename Sched<N>::L>;
+  static constexpr int R = F::R, NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
+  double2 v[K][R];
+};
+
+template <int N, int T, int NT, class LD>
+__device__ __forceinline__ void load_first(Regs<N, T, NT> &r, LD &load) {
+  using RG = Regs<N, T, NT>;
+#pragma unroll
+  for (int k = 0; k < RG::K; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (RG::TOT % NT == 0 || idx < RG::TOT) {
+      const int t = idx % T, bb = idx / T;
+#pragma unroll
+      for (int j = 0; j < RG::R; j++) r.v[k][j] = load(bb + j * RG::NB, t);
+    }
+  }
+}
+
+template <int N, int T, int NT, int SIGN, class ST, int... Rs>
+__device__ __forceinline__ void rest_stages(double2 *lds, const double2 *tw, ST &store, RList<Rs...>) {
+  if constexpr (sizeof...(Rs) > 0) {
+    auto none = [](int, int) { return make_double2(0.0, 0.0); };
+    run_stages<N, T, NT, SIGN, false, Interleaved<N, T>, Regs<N, T, NT>::R, false>(lds, tw, none, store, RList<Rs...>{});
+  }
+}
+
+// Stage 1 from registers (no loads), then the remaining stages through LDS; outputs go to `store`.
+template <int N, int T, int NT, int SIGN, class ST>
+__device__ __forceinline__ void fft_regs(double2 *__restrict__ lds, const double2 *__restrict__ tw, Regs<N, T, NT> &r,
+                                         ST &store) {
+  using RG = Regs<N, T, NT>;
+  constexpr int R = RG::R, S = 1;
+  constexpr bool LAST = RG::F::single;
+#pragma unroll
+  for (int k = 0; k < RG::K; k++) {
+    const int idx = threadIdx.x + k * NT;
+    if (RG::TOT % NT == 0 || idx < RG::TOT) {
+      const int t = idx % T, bb = idx / T;
+      const int p = bb / S, q = bb % S;
+      Dft<R, SIGN>::run(r.v[k]);
+      if (S * R != N) {
+#pragma unroll
+        for (int j = 1; j < R; j++) r.v[k][j] = mul_tw<SIGN>(r.v[k][j], tw[j * p * S]);
+      }
+      const int o0 = q + S * R * p;
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int n = o0 + j * S;
+        if (LAST) store(n, t, r.v[k][j]);
+        else lds[n * T + t] = r.v[k][j];
+      }
+    }
+  }
+  if (!LAST) __syncthreads();
+  rest_stages<N, T, NT, SIGN>(lds, tw, store, typename RG::F::Rest{});
+}
+
+}  // namespace fast
+}  // namespace nsl

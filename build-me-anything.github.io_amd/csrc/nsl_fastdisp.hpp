@@ -1,0 +1,191 @@
+// Host dispatch of the size-specialised pass kernels (nsl_fastk.hpp). Included by nslab_hip.hip.
+// Sizes with a compiled radix schedule run the fast kernels; every other valid N runs the generic engine.
+#pragma once
+#include "nsl_fastk.hpp"
+
+// Lines per workgroup in the pencil passes (T) and row pairs per workgroup in the x pass (LP), per N.
+#ifndef NSL_T_SMALL
+#define NSL_T_SMALL 8
+#endif
+#ifndef NSL_PERSISTENT
+#define NSL_PERSISTENT 0
+#endif
+#ifndef NSL_LP_ELEMS
+#define NSL_LP_ELEMS 4096
+#endif
+static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
+static constexpr int fast_LP(int N) {
+  return 6 * N * 8 <= NSL_LP_ELEMS ? 8 : 6 * N * 4 <= NSL_LP_ELEMS ? 4 : 6 * N * 2 <= NSL_LP_ELEMS ? 2 : 1;
+}
+
+#define NSL_FAST_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+static bool fast_setup(Xform &x) {
+  switch (x.g.N) {
+#define NSL_CASE(n) case n:
+    NSL_FAST_SIZES(NSL_CASE)
+#undef NSL_CASE
+    x.fast = true;
+    x.fT = fast_T(x.g.N);
+    x.fLP = fast_LP(x.g.N);
+    return true;
+    default:
+      x.fast = false;
+      return false;
+  }
+}
+
+static int fast_ntile(const Xform &x) { return (x.g.hx + x.fT - 1) / x.fT; }
+
+static int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+// Persistent grid: every CU holds as many workgroups as its LDS and registers allow, and they stride the tiles.
+template <typename K>
+static unsigned persistent_grid(K kernel, int lds, long long total, int &cache) {
+  if (!cache) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NSL_NT, lds) != hipSuccess || nb < 1) nb = 1;
+    cache = nb * device_cus();
+  }
+  return (unsigned)std::max<long long>(1, std::min<long long>(total, cache));
+}
+
+template <int N, int MODE>
+static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int nout, int gc, const Ctl *ctl) {
+  constexpr int T = fast_T(N);
+  const int lds = N * T * (int)sizeof(double2);
+  allow_lds(fast::kf_zinv<N, T, MODE>, lds);
+  hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+}
+
+template <int N>
+static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
+  constexpr int T = fast_T(N);
+  const int lds = N * T * (int)sizeof(double2);
+  const long long total = (long long)nout * fast_ntile(x) * x.g.by;
+  static int c0 = 0, c1 = 0, c2 = 0;
+  if (!NSL_PERSISTENT) {
+    if (mode == 0) kf_zinv_launch<N, 0>(x, U, X1, nout, gc, ctl);
+    else if (mode == 1) kf_zinv_launch<N, 1>(x, U, X1, nout, gc, ctl);
+    else kf_zinv_launch<N, 2>(x, U, X1, nout, gc, ctl);
+  } else if (mode == 0) {
+    allow_lds(fast::kp_zinv<N, T, 0>, lds);
+    hipLaunchKernelGGL((fast::kp_zinv<N, T, 0>), dim3(persistent_grid(fast::kp_zinv<N, T, 0>, lds, total, c0)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+  } else if (mode == 1) {
+    allow_lds(fast::kp_zinv<N, T, 1>, lds);
+    hipLaunchKernelGGL((fast::kp_zinv<N, T, 1>), dim3(persistent_grid(fast::kp_zinv<N, T, 1>, lds, total, c1)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+  } else {
+    allow_lds(fast::kp_zinv<N, T, 2>, lds);
+    hipLaunchKernelGGL((fast::kp_zinv<N, T, 2>), dim3(persistent_grid(fast::kp_zinv<N, T, 2>, lds, total, c2)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+  }
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+template <int N>
+static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
+  constexpr int T = fast_T(N);
+  const int lds = N * T * (int)sizeof(double2);
+  static int c = 0;
+  if (!NSL_PERSISTENT) {
+    allow_lds(fast::kf_yinv<N, T>, lds);
+    hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X1, X2, x.g, x.twN, ctl);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  allow_lds(fast::kp_yinv<N, T>, lds);
+  const unsigned grid = persistent_grid(fast::kp_yinv<N, T>, lds, (long long)fast_ntile(x) * N * ncomp, c);
+  hipLaunchKernelGGL((fast::kp_yinv<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X1, X2, x.g, x.twN, ncomp, ctl);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+template <int N>
+static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
+  constexpr int T = fast_T(N);
+  const int lds = N * T * (int)sizeof(double2);
+  static int c = 0;
+  if (!NSL_PERSISTENT) {
+    allow_lds(fast::kf_yfwd<N, T>, lds);
+    hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X3, X4, x.g, x.twN, ctl);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  allow_lds(fast::kp_yfwd<N, T>, lds);
+  const unsigned grid = persistent_grid(fast::kp_yfwd<N, T>, lds, (long long)fast_ntile(x) * N * ncomp, c);
+  hipLaunchKernelGGL((fast::kp_yfwd<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X3, X4, x.g, x.twN, ncomp, ctl);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+template <int N>
+static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
+  constexpr int T = fast_T(N);
+  const int lds = N * T * (int)sizeof(double2);
+  const double inv = 1.0 / ((double)N * N * N);
+  static int c = 0;
+  if (!NSL_PERSISTENT) {
+    allow_lds(fast::kf_zfwd<N, T>, lds);
+    hipLaunchKernelGGL((fast::kf_zfwd<N, T>), dim3((unsigned)(fast_ntile(x) * x.g.by), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X4, OUT, x.g, x.twN, inv, ctl);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  allow_lds(fast::kp_zfwd<N, T>, lds);
+  const unsigned grid = persistent_grid(fast::kp_zfwd<N, T>, lds, (long long)fast_ntile(x) * x.g.by * ncomp, c);
+  hipLaunchKernelGGL((fast::kp_zfwd<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X4, OUT, x.g, x.twN, inv, ncomp, ctl);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+template <int N>
+static int fast_xnl_n(const Xform &x, double2 *X2, Ctl *ctl) {
+  constexpr int LP = fast_LP(N);
+  const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
+  const long long npairs = (long long)N * (N / 2);
+  const int nblk = (int)((npairs + LP - 1) / LP);
+  allow_lds(fast::kf_xnl<N, LP>, lds);
+  hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, x.pmax, ctl);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512)), dim3(NSL_NT), 0, x.st, (const double *)x.pmax, nblk, ctl);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+
+#define NSL_DISPATCH(fn, ...)                                          \
+  switch (x.g.N) {                                                     \
+    case 16: return fn<16>(x, __VA_ARGS__);                            \
+    case 24: return fn<24>(x, __VA_ARGS__);                            \
+    case 32: return fn<32>(x, __VA_ARGS__);                            \
+    case 48: return fn<48>(x, __VA_ARGS__);                            \
+    case 64: return fn<64>(x, __VA_ARGS__);                            \
+    case 96: return fn<96>(x, __VA_ARGS__);                            \
+    case 128: return fn<128>(x, __VA_ARGS__);                          \
+    case 192: return fn<192>(x, __VA_ARGS__);                          \
+    case 256: return fn<256>(x, __VA_ARGS__);                          \
+    case 384: return fn<384>(x, __VA_ARGS__);                          \
+    case 512: return fn<512>(x, __VA_ARGS__);                          \
+    case 768: return fn<768>(x, __VA_ARGS__);                          \
+    case 1024: return fn<1024>(x, __VA_ARGS__);                        \
+    default: return fail(NSL_ERR_STATE, "no size-specialised kernels for this N"); \
+  }
+
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
+  NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl)
+}
+static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
+  NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl)
+}
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
+  NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl)
+}
+static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
+  NSL_DISPATCH(fast_zfwd_n, X4, OUT, ncomp, ctl)
+}
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, Ctl *ctl) {
+  NSL_DISPATCH(fast_xnl_n, X2, ctl)
+}

@@ -1,0 +1,391 @@
+// Size-specialised pass kernels (P1..P5 of one rhs) over the compile-time FFT engine in nsl_fast.hpp.
+// Same data layouts and arithmetic as the generic kernels in nsl_kernels.hpp; only the transform engine differs
+// (radix-8/16 stages in registers, first stage read from HBM, last stage written to HBM).
+#pragma once
+#include <type_traits>
+
+#include "nsl_fast.hpp"
+#include "nsl_kernels.hpp"
+
+namespace nsl {
+namespace fast {
+
+// The 2/3-truncated geometry as a compile-time constant of N (these kernels serve the solver state only).
+template <int N>
+__host__ __device__ constexpr Geo trunc_geo() {
+  Geo g{};
+  g.N = N; g.NH = N / 2 + 1; g.hx = N / 3 + 1; g.by = 2 * (N / 3) + 1; g.hy = N / 3 + 1;
+  g.hp = (g.hx + 7) & ~7;
+  g.D = (long long)g.by * g.by * g.hp;
+  return g;
+}
+
+// P1: X[o][z][cy][kx] = z-inverse of field o (MODE 0 stored, 1 u/omega, 2 gradient of gc). Grid ntile*nout*by,
+// tile fastest: neighbouring workgroups write neighbouring segments of the same rows, and the nout workgroups that
+// read the same state tile run close together (their re-reads of the state hit L2).
+template <int N, int T, int MODE>
+__global__ __launch_bounds__(NSL_NT) void kf_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+                                                   const double2 *__restrict__ tw, int nout, int gc,
+                                                   const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = bid % ntile, rest = bid / ntile;
+  const int o = rest % nout, cy = rest / nout, kx0 = tile * T;
+  const double ky = kwv(full_of(cy, g), N);
+  const int plane = g.by * g.hp;
+  const size_t ubase = (size_t)cy * g.hp + kx0;
+  auto load = [&](int n, int t) -> double2 {
+    const int kx = kx0 + t, cz = compact_of(n, g);
+    if (cz < 0 || kx >= g.hx) return make_double2(0.0, 0.0);
+    return zsrc<MODE>(U, g.D, ubase + (size_t)cz * plane + t, o, (double)kx, ky, kwv(n, N), gc);
+  };
+  double2 *Xb = X + ((size_t)o * N * g.by + cy) * g.hp + kx0;
+  auto store = [&](int n, int t, double2 v) {
+    if (kx0 + t < g.hx) Xb[(size_t)n * plane + t] = v;
+  };
+  fft<N, T, NSL_NT, 1, false>(lds, tw, load, store);
+}
+
+// P2: X1[o][z][cy][kx] -> X2[o][z][y][kx]. Grid (ntile*N, ncomp).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kf_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
+                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T;
+  const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
+  const double2 *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
+  double2 *Ob = X2 + ((size_t)o * N + z) * N * g.hp + kx0;
+  auto load = [&](int n, int t) -> double2 {
+    const int cy = compact_of(n, g);
+    return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
+  };
+  auto store = [&](int n, int t, double2 v) {
+    if (kx0 + t < g.hx) Ob[n * g.hp + t] = v;
+  };
+  fft<N, T, NSL_NT, 1, false>(lds, tw, load, store);
+}
+
+// P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
+                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T;
+  const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
+  const double2 *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
+  double2 *Ob = X4 + ((size_t)c * N + z) * g.by * g.hp + kx0;
+  auto load = [&](int n, int t) -> double2 {
+    return kx0 + t < g.hx ? Ib[n * g.hp + t] : make_double2(0.0, 0.0);
+  };
+  auto store = [&](int n, int t, double2 v) {
+    const int cy = compact_of(n, g);
+    if (cy >= 0 && kx0 + t < g.hx) Ob[cy * g.hp + t] = v;
+  };
+  fft<N, T, NSL_NT, -1, false>(lds, tw, load, store);
+}
+
+// P5: X4[c][z][cy][kx] -> OUT[c][cz][cy][kx] * inv (retained kz). Grid (ntile*by, ncomp).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kf_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
+                                                   const double2 *__restrict__ tw, double inv, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T;
+  const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
+  const int plane = g.by * g.hp;
+  const double2 *Ib = X4 + (size_t)c * N * plane + (size_t)cy * g.hp + kx0;
+  double2 *Ob = OUT + (size_t)c * g.D + (size_t)cy * g.hp + kx0;
+  auto load = [&](int n, int t) -> double2 {
+    return kx0 + t < g.hx ? Ib[(size_t)n * plane + t] : make_double2(0.0, 0.0);
+  };
+  auto store = [&](int n, int t, double2 v) {
+    const int cz = compact_of(n, g);
+    if (cz >= 0 && kx0 + t < g.hx) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
+  };
+  fft<N, T, NSL_NT, -1, false>(lds, tw, load, store);
+}
+
+// LDS layout of the x pass (interleaved measured faster than line-major on gfx950).
+#ifndef NSL_XNL_LINEMAJOR
+#define NSL_XNL_LINEMAJOR 0
+#endif
+template <int N, int T>
+using XnlLayout = typename std::conditional<NSL_XNL_LINEMAJOR != 0, LineMajor<N, T>, Interleaved<N, T>>::type;
+
+// P3: in place on X2 (6 fields in, 3 out), LP row pairs per workgroup. Same packing and product as k_xnl.
+template <int N, int LP>
+__global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+                                                  double *__restrict__ pmax, Ctl *__restrict__ ctl) {
+  if (ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int T = 6 * LP, T2 = 3 * LP, half = N / 2, npairs = N * half;
+  using L1 = XnlLayout<N, T>;
+  using L2 = XnlLayout<N, T2>;
+  // retained kx modes and row pitch are functions of N (the 2/3 rule); compile-time so the loops below unroll and
+  // every load of the workgroup is in flight at once
+  constexpr int hx = N / 3 + 1, HP = (hx + 7) & ~7;
+  constexpr int ITEMS = T * hx, KL = (ITEMS + NSL_NT - 1) / NSL_NT;
+  constexpr size_t cs = (size_t)N * N * HP;
+  double2 A[KL], B[KL];
+#pragma unroll
+  for (int i = 0; i < KL; i++) {
+    const int e = threadIdx.x + i * NSL_NT;
+    A[i] = B[i] = make_double2(0.0, 0.0);
+    if (ITEMS % NSL_NT == 0 || e < ITEMS) {
+      const int t = e / hx, k = e - t * hx, p = t / 6, o = t - p * 6;
+      const int pr = blockIdx.x * LP + p;
+      if (pr < npairs) {
+        const int z = pr / half, y = 2 * (pr - z * half);
+        const size_t base = o * cs + ((size_t)z * N + y) * HP;
+        A[i] = X2[base + k];
+        B[i] = X2[base + HP + k];
+      }
+    }
+  }
+  // modes hx .. N-hx of every line are zero
+  for (int e = threadIdx.x; e < T * (N - 2 * hx + 1); e += NSL_NT) {
+    const int t = e / (N - 2 * hx + 1), n = hx + (e - t * (N - 2 * hx + 1));
+    lds[L1::at(n, t)] = make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < KL; i++) {
+    const int e = threadIdx.x + i * NSL_NT;
+    if (ITEMS % NSL_NT == 0 || e < ITEMS) {
+      const int t = e / hx, k = e - t * hx;
+      lds[L1::at(k, t)] = make_double2(A[i].x - B[i].y, A[i].y + B[i].x);
+      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = make_double2(A[i].x + B[i].y, B[i].x - A[i].y);
+    }
+  }
+  __syncthreads();
+  auto ldl = [&](int n, int t) -> double2 { return lds[L1::at(n, t)]; };
+  auto stl = [&](int n, int t, double2 v) { lds[L1::at(n, t)] = v; };
+  fft<N, T, NSL_NT, 1, true, L1>(lds, tw, ldl, stl);
+  __syncthreads();
+  constexpr int KI = (N * LP + NSL_NT - 1) / NSL_NT;
+  double nv[KI][6];
+  double om2 = 0.0, u2 = 0.0;
+#pragma unroll
+  for (int k2 = 0; k2 < KI; k2++) {
+    const int it = threadIdx.x + k2 * NSL_NT;
+    if ((N * LP) % NSL_NT == 0 || it < N * LP) {
+      const int x = it % N, p = it / N;
+      const double2 Za = lds[L1::at(x, p * 6)], Zb = lds[L1::at(x, p * 6 + 1)], Zc = lds[L1::at(x, p * 6 + 2)];
+      const double2 Zp = lds[L1::at(x, p * 6 + 3)], Zq = lds[L1::at(x, p * 6 + 4)], Zr = lds[L1::at(x, p * 6 + 5)];
+      {
+        const double a = Za.x, b = Zb.x, c = Zc.x, pp = Zp.x, q = Zq.x, r = Zr.x;
+        const double o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
+        om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+        nv[k2][0] = b * r - c * q; nv[k2][1] = c * pp - a * r; nv[k2][2] = a * q - b * pp;
+      }
+      {
+        const double a = Za.y, b = Zb.y, c = Zc.y, pp = Zp.y, q = Zq.y, r = Zr.y;
+        const double o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
+        om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+        nv[k2][3] = b * r - c * q; nv[k2][4] = c * pp - a * r; nv[k2][5] = a * q - b * pp;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < KI; k2++) {
+    const int it = threadIdx.x + k2 * NSL_NT;
+    if ((N * LP) % NSL_NT == 0 || it < N * LP) {
+      const int x = it % N, p = it / N;
+      lds[L2::at(x, p * 3)] = make_double2(nv[k2][0], nv[k2][3]);
+      lds[L2::at(x, p * 3 + 1)] = make_double2(nv[k2][1], nv[k2][4]);
+      lds[L2::at(x, p * 3 + 2)] = make_double2(nv[k2][2], nv[k2][5]);
+    }
+  }
+  __syncthreads();
+  auto ldl2 = [&](int n, int t) -> double2 { return lds[L2::at(n, t)]; };
+  auto stl2 = [&](int n, int t, double2 v) { lds[L2::at(n, t)] = v; };
+  fft<N, T2, NSL_NT, -1, true, L2>(lds, tw, ldl2, stl2);
+  __syncthreads();
+  constexpr int ITEMS2 = T2 * hx, KS = (ITEMS2 + NSL_NT - 1) / NSL_NT;
+#pragma unroll
+  for (int i = 0; i < KS; i++) {
+    const int e = threadIdx.x + i * NSL_NT;
+    if (!(ITEMS2 % NSL_NT == 0 || e < ITEMS2)) continue;
+    const int t = e / hx, k = e - t * hx, p = t / 3, c = t - p * 3;
+    const int pr = blockIdx.x * LP + p;
+    if (pr >= npairs) continue;
+    const int z = pr / half, y = 2 * (pr - z * half);
+    const double2 Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
+    const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+    const double2 X = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+    const double2 Y = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+    const size_t base = c * cs + ((size_t)z * N + y) * HP;
+    X2[base + k] = X;
+    X2[base + HP + k] = Y;
+  }
+  block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds);
+}
+
+// Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
+// workgroups, one atomic pair each (the max is exact, so the order does not matter).
+__global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl) {
+  __shared__ double scratch[8];
+  if (ctl->active == 0) return;
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * NSL_NT + threadIdx.x; i < nblk; i += gridDim.x * NSL_NT) {
+    a = fmax(a, pmax[2 * (size_t)i]);
+    b = fmax(b, pmax[2 * (size_t)i + 1]);
+  }
+  block_atomic_max2(a, b, &ctl->om2, &ctl->u2, scratch);
+}
+
+}  // namespace fast
+}  // namespace nsl
+
+// ------------------------------------------------------------------------------------------------ persistent forms
+// A grid of (CUs x resident workgroups) walks the tiles of a pass; each workgroup alternates two register sets so
+// the loads of its next tile are in flight while it transforms the current one.
+namespace nsl {
+namespace fast {
+
+template <int N, int T, class LoadT, class RunT>
+__device__ __forceinline__ void pipeline(int total, LoadT &loadTile, RunT &runTile) {
+  Regs<N, T, NSL_NT> A, B;
+  int it = blockIdx.x;
+  if (it >= total) return;
+  loadTile(A, it);
+  while (true) {
+    const int i1 = it + gridDim.x;
+    if (i1 < total) loadTile(B, i1);
+    runTile(A, it);
+    if (i1 >= total) break;
+    const int i2 = i1 + gridDim.x;
+    if (i2 < total) loadTile(A, i2);
+    runTile(B, i1);
+    if (i2 >= total) break;
+    it = i2;
+  }
+}
+
+// P1 persistent: tile index it = o + nout*(tile + ntile*cy).
+template <int N, int T, int MODE>
+__global__ __launch_bounds__(NSL_NT) void kp_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+                                                   const double2 *__restrict__ tw, int nout, int gc,
+                                                   const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T, total = nout * ntile * g.by, plane = g.by * g.hp;
+  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, rest = it / ntile, o = rest % nout, cy = rest / nout, kx0 = tile * T;
+    const double ky = kwv(full_of(cy, g), N);
+    const size_t ubase = (size_t)cy * g.hp + kx0;
+    auto load = [&](int n, int t) -> double2 {
+      const int kx = kx0 + t, cz = compact_of(n, g);
+      if (cz < 0 || kx >= g.hx) return make_double2(0.0, 0.0);
+      return zsrc<MODE>(U, g.D, ubase + (size_t)cz * plane + t, o, (double)kx, ky, kwv(n, N), gc);
+    };
+    load_first<N, T, NSL_NT>(r, load);
+  };
+  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, rest = it / ntile, o = rest % nout, cy = rest / nout, kx0 = tile * T;
+    double2 *Xb = X + ((size_t)o * N * g.by + cy) * g.hp + kx0;
+    auto store = [&](int n, int t, double2 v) {
+      if (kx0 + t < g.hx) Xb[(size_t)n * plane + t] = v;
+    };
+    fft_regs<N, T, NSL_NT, 1>(lds, tw, r, store);
+  };
+  pipeline<N, T>(total, loadTile, runTile);
+}
+
+// P2 persistent: it = tile + ntile*(z + N*o).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kp_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
+                                                   const double2 *__restrict__ tw, int ncomp, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T, total = ntile * N * ncomp;
+  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, zo = it / ntile, kx0 = tile * T;
+    const double2 *Ib = X1 + (size_t)zo * g.by * g.hp + kx0;
+    auto load = [&](int n, int t) -> double2 {
+      const int cy = compact_of(n, g);
+      return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
+    };
+    load_first<N, T, NSL_NT>(r, load);
+  };
+  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, zo = it / ntile, kx0 = tile * T;
+    double2 *Ob = X2 + (size_t)zo * N * g.hp + kx0;
+    auto store = [&](int n, int t, double2 v) {
+      if (kx0 + t < g.hx) Ob[n * g.hp + t] = v;
+    };
+    fft_regs<N, T, NSL_NT, 1>(lds, tw, r, store);
+  };
+  pipeline<N, T>(total, loadTile, runTile);
+}
+
+// P4 persistent: it = tile + ntile*(z + N*c).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kp_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
+                                                   const double2 *__restrict__ tw, int ncomp, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T, total = ntile * N * ncomp;
+  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, zc = it / ntile, kx0 = tile * T;
+    const double2 *Ib = X3 + (size_t)zc * N * g.hp + kx0;
+    auto load = [&](int n, int t) -> double2 {
+      return kx0 + t < g.hx ? Ib[n * g.hp + t] : make_double2(0.0, 0.0);
+    };
+    load_first<N, T, NSL_NT>(r, load);
+  };
+  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, zc = it / ntile, kx0 = tile * T;
+    double2 *Ob = X4 + (size_t)zc * g.by * g.hp + kx0;
+    auto store = [&](int n, int t, double2 v) {
+      const int cy = compact_of(n, g);
+      if (cy >= 0 && kx0 + t < g.hx) Ob[cy * g.hp + t] = v;
+    };
+    fft_regs<N, T, NSL_NT, -1>(lds, tw, r, store);
+  };
+  pipeline<N, T>(total, loadTile, runTile);
+}
+
+// P5 persistent: it = tile + ntile*(cy + by*c).
+template <int N, int T>
+__global__ __launch_bounds__(NSL_NT) void kp_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
+                                                   const double2 *__restrict__ tw, double inv, int ncomp,
+                                                   const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  const int ntile = (g.hx + T - 1) / T, total = ntile * g.by * ncomp, plane = g.by * g.hp;
+  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, rest = it / ntile, cy = rest % g.by, c = rest / g.by, kx0 = tile * T;
+    const double2 *Ib = X4 + (size_t)c * N * plane + (size_t)cy * g.hp + kx0;
+    auto load = [&](int n, int t) -> double2 {
+      return kx0 + t < g.hx ? Ib[(size_t)n * plane + t] : make_double2(0.0, 0.0);
+    };
+    load_first<N, T, NSL_NT>(r, load);
+  };
+  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
+    const int tile = it % ntile, rest = it / ntile, cy = rest % g.by, c = rest / g.by, kx0 = tile * T;
+    double2 *Ob = OUT + (size_t)c * g.D + (size_t)cy * g.hp + kx0;
+    auto store = [&](int n, int t, double2 v) {
+      const int cz = compact_of(n, g);
+      if (cz >= 0 && kx0 + t < g.hx) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
+    };
+    fft_regs<N, T, NSL_NT, -1>(lds, tw, r, store);
+  };
+  pipeline<N, T>(total, loadTile, runTile);
+}
+
+}  // namespace fast
+}  // namespace nsl

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(NSL_NT) void k_zinv(const double2 *__restrict__ U, 
     double2 v = make_double2(0.0, 0.0);
     const int cz = compact_of(n, g);
     if (cz >= 0 && kx < g.hx) {
-      const size_t m = ((size_t)cz * g.by + cy) * g.hx + kx;
+      const size_t m = ((size_t)cz * g.by + cy) * g.hp + kx;
       v = zsrc<MODE>(U, g.D, m, o, (double)kx, ky, kwv(n, N), gc);
     }
     lds[e] = v;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(NSL_NT) void k_zinv(const double2 *__restrict__ U, 
   lds_fft<TN>(lds, T, P, 1.0);
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
-    if (kx < g.hx) X[(((size_t)o * N + n) * g.by + cy) * g.hx + kx] = lds[e];
+    if (kx < g.hx) X[(((size_t)o * N + n) * g.by + cy) * g.hp + kx] = lds[e];
   }
 }
 
@@ -122,13 +122,13 @@ __global__ __launch_bounds__(NSL_NT) void k_yinv(const double2 *__restrict__ X1,
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
     const int cy = compact_of(n, g);
-    lds[e] = (cy >= 0 && kx < g.hx) ? X1[(inb + cy) * g.hx + kx] : make_double2(0.0, 0.0);
+    lds[e] = (cy >= 0 && kx < g.hx) ? X1[(inb + cy) * g.hp + kx] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   lds_fft<TN>(lds, T, P, 1.0);
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
-    if (kx < g.hx) X2[(outb + n) * g.hx + kx] = lds[e];
+    if (kx < g.hx) X2[(outb + n) * g.hp + kx] = lds[e];
   }
 }
 
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(NSL_NT) void k_xnl(double2 *__restrict__ X2, const 
   extern __shared__ double2 lds[];
   const int N = g.N, hx = g.hx, half = N >> 1, T = 6 * LP, T2 = 3 * LP;
   const int npairs = N * half;
-  const size_t cs = (size_t)N * N * hx;   // component stride
+  const size_t cs = (size_t)N * N * g.hp;   // component stride
   for (int e = threadIdx.x; e < N * T; e += NSL_NT) lds[e] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int e = threadIdx.x; e < T * hx; e += NSL_NT) {
@@ -165,8 +165,8 @@ __global__ __launch_bounds__(NSL_NT) void k_xnl(double2 *__restrict__ X2, const 
     const int pr = blockIdx.x * LP + p;
     if (pr >= npairs) continue;
     const int z = pr / half, y = 2 * (pr - z * half);
-    const size_t base = o * cs + ((size_t)z * N + y) * hx;
-    pack_pair(lds, T, t, k, N, X2[base + k], X2[base + hx + k]);
+    const size_t base = o * cs + ((size_t)z * N + y) * g.hp;
+    pack_pair(lds, T, t, k, N, X2[base + k], X2[base + g.hp + k]);
   }
   __syncthreads();
   lds_fft<TN>(lds, T, P, 1.0);
@@ -215,9 +215,9 @@ __global__ __launch_bounds__(NSL_NT) void k_xnl(double2 *__restrict__ X2, const 
     const int z = pr / half, y = 2 * (pr - z * half);
     double2 X, Y;
     unpack_pair(lds, T2, t, k, N, X, Y);
-    const size_t base = c * cs + ((size_t)z * N + y) * hx;
+    const size_t base = c * cs + ((size_t)z * N + y) * g.hp;
     X2[base + k] = X;
-    X2[base + hx + k] = Y;
+    X2[base + g.hp + k] = Y;
   }
   block_atomic_max2(om2, u2, &ctl->om2, &ctl->u2, (double *)lds);
 }
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(NSL_NT) void k_xfwd(const double *__restrict__ phys
   extern __shared__ double2 lds[];
   const int N = g.N, hx = g.hx, half = N >> 1, T = ncomp * LP;
   const int npairs = N * half;
-  const size_t ps = (size_t)N * N * N, cs = (size_t)N * N * hx;
+  const size_t ps = (size_t)N * N * N, cs = (size_t)N * N * g.hp;
   for (int e = threadIdx.x; e < N * T; e += NSL_NT) {
     const int x = e / T, t = e - x * T, p = t / ncomp, c = t - p * ncomp;
     const int pr = blockIdx.x * LP + p;
@@ -250,9 +250,9 @@ __global__ __launch_bounds__(NSL_NT) void k_xfwd(const double *__restrict__ phys
     const int z = pr / half, y = 2 * (pr - z * half);
     double2 A, B;
     unpack_pair(lds, T, t, k, N, A, B);
-    const size_t base = c * cs + ((size_t)z * N + y) * hx;
+    const size_t base = c * cs + ((size_t)z * N + y) * g.hp;
     X[base + k] = A;
-    X[base + hx + k] = B;
+    X[base + g.hp + k] = B;
   }
 }
 
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(NSL_NT) void k_xinv(const double2 *__restrict__ X, 
   extern __shared__ double2 lds[];
   const int N = g.N, hx = g.hx, half = N >> 1, T = ncomp * LP;
   const int npairs = N * half;
-  const size_t ps = (size_t)N * N * N, cs = (size_t)N * N * hx;
+  const size_t ps = (size_t)N * N * N, cs = (size_t)N * N * g.hp;
   for (int e = threadIdx.x; e < N * T; e += NSL_NT) lds[e] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int e = threadIdx.x; e < T * hx; e += NSL_NT) {
@@ -271,8 +271,8 @@ __global__ __launch_bounds__(NSL_NT) void k_xinv(const double2 *__restrict__ X, 
     const int pr = blockIdx.x * LP + p;
     if (pr >= npairs) continue;
     const int z = pr / half, y = 2 * (pr - z * half);
-    const size_t base = c * cs + ((size_t)z * N + y) * hx;
-    pack_pair(lds, T, t, k, N, X[base + k], X[base + hx + k]);
+    const size_t base = c * cs + ((size_t)z * N + y) * g.hp;
+    pack_pair(lds, T, t, k, N, X[base + k], X[base + g.hp + k]);
   }
   __syncthreads();
   lds_fft<TN>(lds, T, P, 1.0);
@@ -302,14 +302,14 @@ __global__ __launch_bounds__(NSL_NT) void k_yfwd(const double2 *__restrict__ X3,
   const size_t inb = ((size_t)c * N + z) * N, outb = ((size_t)c * N + z) * g.by;
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
-    lds[e] = kx < g.hx ? X3[(inb + n) * g.hx + kx] : make_double2(0.0, 0.0);
+    lds[e] = kx < g.hx ? X3[(inb + n) * g.hp + kx] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   lds_fft<TN>(lds, T, P, -1.0);
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
     const int cy = compact_of(n, g);
-    if (cy >= 0 && kx < g.hx) X4[(outb + cy) * g.hx + kx] = lds[e];
+    if (cy >= 0 && kx < g.hx) X4[(outb + cy) * g.hp + kx] = lds[e];
   }
 }
 
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(NSL_NT) void k_zfwd(const double2 *__restrict__ X4,
   const int kx0 = tile << lgT;
   for (int e = threadIdx.x; e < (N << lgT); e += NSL_NT) {
     const int n = e >> lgT, t = e & (T - 1), kx = kx0 + t;
-    lds[e] = kx < g.hx ? X4[(((size_t)c * N + n) * g.by + cy) * g.hx + kx] : make_double2(0.0, 0.0);
+    lds[e] = kx < g.hx ? X4[(((size_t)c * N + n) * g.by + cy) * g.hp + kx] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   lds_fft<TN>(lds, T, P, -1.0);
@@ -336,17 +336,19 @@ __global__ __launch_bounds__(NSL_NT) void k_zfwd(const double2 *__restrict__ X4,
     const int cz = compact_of(n, g);
     if (cz >= 0 && kx < g.hx) {
       const double2 v = lds[e];
-      OUT[(size_t)c * g.D + ((size_t)cz * g.by + cy) * g.hx + kx] = make_double2(v.x * inv, v.y * inv);
+      OUT[(size_t)c * g.D + ((size_t)cz * g.by + cy) * g.hp + kx] = make_double2(v.x * inv, v.y * inv);
     }
   }
 }
 
 // ------------------------------------------------------------------------------------------------ spectral sweep
-__device__ __forceinline__ void mode_k(long long m, const Geo &g, double &kx, double &ky, double &kz) {
-  const int ix = (int)(m % g.hx);
-  const long long r = m / g.hx;
+// Wavenumbers of compact slot m; false for the padding slots that round each kx row up to the pitch hp.
+__device__ __forceinline__ bool mode_k(long long m, const Geo &g, double &kx, double &ky, double &kz) {
+  const int ix = (int)(m % g.hp);
+  const long long r = m / g.hp;
   const int cy = (int)(r % g.by), cz = (int)(r / g.by);
   kx = (double)ix; ky = kwv(full_of(cy, g), g.N); kz = kwv(full_of(cz, g), g.N);
+  return ix < g.hx;
 }
 
 #define NSL_MPT 4   // modes per thread in the spectral sweeps
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ N
     const long long m = m0 + (long long)q * NSL_NT;
     if (m >= D) break;
     double kx, ky, kz;
-    mode_k(m, g, kx, ky, kz);
+    if (!mode_k(m, g, kx, ky, kz)) continue;
     const double k2 = kx * kx + ky * ky + kz * kz;
     double2 n0 = NL[m], n1 = NL[D + m], n2 = NL[2 * D + m];
     if (k2 == 0) {
@@ -520,7 +522,7 @@ __global__ void k_project(double2 *__restrict__ U, const Geo g) {
   const long long D = g.D;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < D; m += (long long)gridDim.x * blockDim.x) {
     double kx, ky, kz;
-    mode_k(m, g, kx, ky, kz);
+    if (!mode_k(m, g, kx, ky, kz)) continue;
     const double k2 = kx * kx + ky * ky + kz * kz;
     double2 a = U[m], b = U[D + m], c = U[2 * D + m];
     project_mode(kx, ky, kz, k2, a, b, c);
@@ -533,7 +535,7 @@ __global__ void k_tubes_finish(const double2 *__restrict__ W, double2 *__restric
   const long long D = g.D;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < D; m += (long long)gridDim.x * blockDim.x) {
     double kx, ky, kz;
-    mode_k(m, g, kx, ky, kz);
+    if (!mode_k(m, g, kx, ky, kz)) continue;
     double k2 = kx * kx + ky * ky + kz * kz;
     double2 a = W[m], b = W[D + m], c = W[2 * D + m];
     project_mode(kx, ky, kz, k2, a, b, c);
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(NSL_NT) void k_norms(const double2 *__restrict__ U,
     const long long m = m0 + (long long)q * NSL_NT;
     if (m >= D) break;
     double kx, ky, kz;
-    mode_k(m, g, kx, ky, kz);
+    if (!mode_k(m, g, kx, ky, kz)) continue;
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double2 a = U[m], b = U[D + m], c = U[2 * D + m];
     const double h = (kx == 0 || 2 * kx == g.N) ? 1.0 : 2.0;

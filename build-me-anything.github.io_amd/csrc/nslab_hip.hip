@@ -40,6 +40,12 @@ static int fail(int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(NSL_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
 
+#define TRY(expr)                \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != NSL_OK) return r_; \
+  } while (0)
+
 static const double TWO_PI = 6.283185307179586;
 
 // ------------------------------------------------------------------------------------------------ plans
@@ -124,19 +130,33 @@ struct Xform {
   HostPlan hp;
   Tiling tl;
   hipStream_t st = 0;
+  // size-specialised engine (nsl_fastk.hpp) when N has a compiled schedule
+  bool fast = false;
+  int fT = 0, fLP = 0;
+  double2 *twN = nullptr;   // (cos, sin)(2 pi k / N), k < N
+  double *pmax = nullptr;   // per-workgroup max |omega|^2, |u|^2 of the x pass
 };
+
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl);
+static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl);
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl);
+static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl);
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, Ctl *ctl);
+static bool fast_setup(Xform &x);
 
 static Geo geo_truncated(int N) {
   Geo g;
   const int kc = N / 3;
   g.N = N; g.NH = N / 2 + 1; g.hx = kc + 1; g.by = 2 * kc + 1; g.hy = kc + 1;
-  g.D = (long long)g.by * g.by * g.hx;
+  g.hp = (g.hx + 7) & ~7;   // rows start on 128-byte boundaries
+  g.D = (long long)g.by * g.by * g.hp;
   return g;
 }
 static Geo geo_full(int N) {
   Geo g;
   g.N = N; g.NH = N / 2 + 1; g.hx = g.NH; g.by = N; g.hy = N / 2;
-  g.D = (long long)g.by * g.by * g.hx;
+  g.hp = g.hx;              // the reference layout, unpadded
+  g.D = (long long)g.by * g.by * g.hp;
   return g;
 }
 
@@ -153,6 +173,7 @@ static int zinv_t(const Xform &x, const double2 *U, double2 *X1, int mode, int n
   return NSL_OK;
 }
 static int zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
+  if (x.fast) return fast_dispatch_zinv(x, U, X1, mode, nout, gc, ctl);
   return x.tl.TNp == 4096 ? zinv_t<4096>(x, U, X1, mode, nout, gc, ctl) : zinv_t<8192>(x, U, X1, mode, nout, gc, ctl);
 }
 
@@ -165,6 +186,7 @@ static int yinv_t(const Xform &x, const double2 *X1, double2 *X2, int ncomp, con
   return NSL_OK;
 }
 static int yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
+  if (x.fast) return fast_dispatch_yinv(x, X1, X2, ncomp, ctl);
   return x.tl.TNp == 4096 ? yinv_t<4096>(x, X1, X2, ncomp, ctl) : yinv_t<8192>(x, X1, X2, ncomp, ctl);
 }
 
@@ -177,6 +199,7 @@ static int yfwd_t(const Xform &x, const double2 *X3, double2 *X4, int ncomp, con
   return NSL_OK;
 }
 static int yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
+  if (x.fast) return fast_dispatch_yfwd(x, X3, X4, ncomp, ctl);
   return x.tl.TNp == 4096 ? yfwd_t<4096>(x, X3, X4, ncomp, ctl) : yfwd_t<8192>(x, X3, X4, ncomp, ctl);
 }
 
@@ -190,6 +213,7 @@ static int zfwd_t(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, co
   return NSL_OK;
 }
 static int zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
+  if (x.fast) return fast_dispatch_zfwd(x, X4, OUT, ncomp, ctl);
   return x.tl.TNp == 4096 ? zfwd_t<4096>(x, X4, OUT, ncomp, ctl) : zfwd_t<8192>(x, X4, OUT, ncomp, ctl);
 }
 
@@ -207,6 +231,7 @@ static int xnl_t(const Xform &x, double2 *X2, Ctl *ctl) {
   return NSL_OK;
 }
 static int xnl(const Xform &x, double2 *X2, Ctl *ctl) {
+  if (x.fast) return fast_dispatch_xnl(x, X2, ctl);
   return x.tl.TNx == 4096 ? xnl_t<4096>(x, X2, ctl) : xnl_t<8192>(x, X2, ctl);
 }
 
@@ -238,12 +263,31 @@ static int xform_init(Xform &x, const Geo &g, hipStream_t st) {
   x.g = g;
   x.st = st;
   if (!tiling_for(g.N, x.tl)) return fail(NSL_ERR_ARG, "grid size " + std::to_string(g.N) + " exceeds the LDS tiling of this build (N <= 1024)");
-  return plan_create(g.N, x.hp);
+  TRY(plan_create(g.N, x.hp));
+  // the size-specialised kernels hard-code the 2/3-truncated geometry; untruncated transforms use the generic ones
+  if (g.by != g.N && fast_setup(x)) {
+    std::vector<double2> tw(g.N);
+    for (int k = 0; k < g.N; k++) {
+      const double a = TWO_PI * k / g.N;
+      tw[k] = make_double2(cos(a), sin(a));
+    }
+    HIP_TRY(hipMalloc(&x.twN, g.N * sizeof(double2)));
+    HIP_TRY(hipMemcpy(x.twN, tw.data(), g.N * sizeof(double2), hipMemcpyHostToDevice));
+    const size_t nblk = ((size_t)g.N * (g.N / 2) + x.fLP - 1) / x.fLP;
+    HIP_TRY(hipMalloc(&x.pmax, 2 * nblk * sizeof(double)));
+  }
+  return NSL_OK;
 }
 static void xform_free(Xform &x) {
   if (x.hp.twd) (void)hipFree(x.hp.twd);
+  if (x.twN) (void)hipFree(x.twN);
+  if (x.pmax) (void)hipFree(x.pmax);
   x.hp.twd = nullptr;
+  x.twN = nullptr;
+  x.pmax = nullptr;
 }
+
+#include "nsl_fastdisp.hpp"
 
 // ------------------------------------------------------------------------------------------------ solver
 #define NSL_RECCAP 64
@@ -319,11 +363,6 @@ static int dalloc(nsl_solver *s, T **p, size_t count) {
   return NSL_OK;
 }
 
-#define TRY(expr)            \
-  do {                       \
-    int r_ = (expr);         \
-    if (r_ != NSL_OK) return r_; \
-  } while (0)
 
 static int prof_collect(nsl_solver *s);
 static int pull_ctl(nsl_solver *s) {
@@ -445,7 +484,7 @@ static int ic_random(nsl_solver *s) {
         if (k2 == 0) continue;
         if (i == 0 && (kz < 0 || (kz == 0 && ky < 0))) continue;
         const double k = sqrt(k2), amp = sqrt(pow(k, 4) * exp(-2 * (k / k0) * (k / k0)) / (4 * M_PI * k2));
-        const size_t m = ((size_t)cz * g.by + cy) * g.hx + i;
+        const size_t m = ((size_t)cz * g.by + cy) * g.hp + i;
         for (int c = 0; c < 3; c++) {
           double v[2];
           for (int part = 0; part < 2; part++) {
@@ -458,7 +497,7 @@ static int ic_random(nsl_solver *s) {
         }
         if (i == 0) {
           const int czm = compact_of_h((N - l) % N, g), cym = compact_of_h((N - j) % N, g);
-          const size_t mm = ((size_t)czm * g.by + cym) * g.hx;
+          const size_t mm = ((size_t)czm * g.by + cym) * g.hp;
           for (int c = 0; c < 3; c++) {
             const double2 v = h[(size_t)c * g.D + m];
             h[(size_t)c * g.D + mm] = make_double2(v.x, -v.y);
@@ -545,7 +584,7 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
   if (r != NSL_OK) { solver_free(s); return r; }
   const Geo &g = s->x.g;
   const size_t D3 = (size_t)3 * g.D;
-  const size_t X1n = (size_t)6 * N * g.by * g.hx, X2n = (size_t)6 * N * N * g.hx;
+  const size_t X1n = (size_t)6 * N * g.by * g.hp, X2n = (size_t)6 * N * N * g.hp;
   s->nblk = (int)((g.D + (long long)NSL_NT * NSL_MPT - 1) / ((long long)NSL_NT * NSL_MPT));
   if ((r = dalloc(s, &s->S, D3)) || (r = dalloc(s, &s->T, D3)) || (r = dalloc(s, &s->ACC, D3)) ||
       (r = dalloc(s, &s->NL, D3)) || (r = dalloc(s, &s->X1, X1n)) || (r = dalloc(s, &s->X2, X2n)) ||
@@ -554,6 +593,9 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
     solver_free(s);
     return r;
   }
+  // the padding slots of every row stay zero for the life of the solver
+  for (double2 *p : {s->S, s->T, s->ACC, s->NL})
+    if (hipMemset(p, 0, D3 * sizeof(double2)) != hipSuccess) { solver_free(s); return fail(NSL_ERR_HIP, "hipMemset failed"); }
   if (hipHostMalloc((void **)&s->hrecs, NSL_RECCAP * sizeof(StepRecDev)) != hipSuccess ||
       hipHostMalloc((void **)&s->hctl, sizeof(Ctl)) != hipSuccess) {
     solver_free(s);
@@ -618,7 +660,7 @@ static int get_state_impl(nsl_solver *s, double *const planes[6]) {
     const int l = full_of_h(cz, g);
     for (int cy = 0; cy < g.by; cy++) {
       const int j = full_of_h(cy, g);
-      const size_t mrow = ((size_t)cz * g.by + cy) * g.hx, frow = ((size_t)l * N + j) * NH;
+      const size_t mrow = ((size_t)cz * g.by + cy) * g.hp, frow = ((size_t)l * N + j) * NH;
       for (int c = 0; c < 3; c++)
         for (int i = 0; i < g.hx; i++) {
           const double2 v = h[(size_t)c * g.D + mrow + i];
@@ -638,7 +680,7 @@ static int set_state_impl(nsl_solver *s, const double *const planes[6]) {
     const int l = full_of_h(cz, g);
     for (int cy = 0; cy < g.by; cy++) {
       const int j = full_of_h(cy, g);
-      const size_t mrow = ((size_t)cz * g.by + cy) * g.hx, frow = ((size_t)l * N + j) * NH;
+      const size_t mrow = ((size_t)cz * g.by + cy) * g.hp, frow = ((size_t)l * N + j) * NH;
       for (int c = 0; c < 3; c++)
         for (int i = 0; i < g.hx; i++)
           h[(size_t)c * g.D + mrow + i] = make_double2(planes[2 * c][frow + i], planes[2 * c + 1][frow + i]);
