@@ -147,8 +147,8 @@ static int fast_xnl_n(const Xform &x, double2 *X2, Ctl *ctl) {
   constexpr int LP = fast_LP(N);
   const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
   const long long npairs = (long long)N * (N / 2);
-  const int nblk = (int)((npairs + LP - 1) / LP);
   allow_lds(fast::kf_xnl<N, LP>, lds);
+  const int nblk = (int)((npairs + LP - 1) / LP);
   hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, x.pmax, ctl);
   LAUNCH_CHECK();
   hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512)), dim3(NSL_NT), 0, x.st, (const double *)x.pmax, nblk, ctl);

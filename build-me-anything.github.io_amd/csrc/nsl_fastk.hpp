@@ -135,22 +135,31 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   constexpr int hx = N / 3 + 1, HP = (hx + 7) & ~7;
   constexpr int ITEMS = T * hx, KL = (ITEMS + NSL_NT - 1) / NSL_NT;
   constexpr size_t cs = (size_t)N * N * HP;
+  constexpr int ngroups = (npairs + LP - 1) / LP;
+  // one group of LP row pairs per workgroup (a persistent loop with prefetch needs 256+ VGPRs on gfx950 and
+  // measured slower)
   double2 A[KL], B[KL];
+  auto load_group = [&](int grp) {
 #pragma unroll
-  for (int i = 0; i < KL; i++) {
-    const int e = threadIdx.x + i * NSL_NT;
-    A[i] = B[i] = make_double2(0.0, 0.0);
-    if (ITEMS % NSL_NT == 0 || e < ITEMS) {
-      const int t = e / hx, k = e - t * hx, p = t / 6, o = t - p * 6;
-      const int pr = blockIdx.x * LP + p;
-      if (pr < npairs) {
-        const int z = pr / half, y = 2 * (pr - z * half);
-        const size_t base = o * cs + ((size_t)z * N + y) * HP;
-        A[i] = X2[base + k];
-        B[i] = X2[base + HP + k];
+    for (int i = 0; i < KL; i++) {
+      const int e = threadIdx.x + i * NSL_NT;
+      A[i] = B[i] = make_double2(0.0, 0.0);
+      if (ITEMS % NSL_NT == 0 || e < ITEMS) {
+        const int t = e / hx, k = e - t * hx, p = t / 6, o = t - p * 6;
+        const int pr = grp * LP + p;
+        if (pr < npairs) {
+          const int z = pr / half, y = 2 * (pr - z * half);
+          const size_t base = o * cs + ((size_t)z * N + y) * HP;
+          A[i] = X2[base + k];
+          B[i] = X2[base + HP + k];
+        }
       }
     }
-  }
+  };
+  double om2 = 0.0, u2 = 0.0;
+  const int grp = blockIdx.x;
+  load_group(grp);
+  {
   // modes hx .. N-hx of every line are zero
   for (int e = threadIdx.x; e < T * (N - 2 * hx + 1); e += NSL_NT) {
     const int t = e / (N - 2 * hx + 1), n = hx + (e - t * (N - 2 * hx + 1));
@@ -172,7 +181,6 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   __syncthreads();
   constexpr int KI = (N * LP + NSL_NT - 1) / NSL_NT;
   double nv[KI][6];
-  double om2 = 0.0, u2 = 0.0;
 #pragma unroll
   for (int k2 = 0; k2 < KI; k2++) {
     const int it = threadIdx.x + k2 * NSL_NT;
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
     const int e = threadIdx.x + i * NSL_NT;
     if (!(ITEMS2 % NSL_NT == 0 || e < ITEMS2)) continue;
     const int t = e / hx, k = e - t * hx, p = t / 3, c = t - p * 3;
-    const int pr = blockIdx.x * LP + p;
+    const int pr = grp * LP + p;
     if (pr >= npairs) continue;
     const int z = pr / half, y = 2 * (pr - z * half);
     const double2 Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
@@ -226,6 +234,7 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
     const size_t base = c * cs + ((size_t)z * N + y) * HP;
     X2[base + k] = X;
     X2[base + HP + k] = Y;
+  }
   }
   block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds);
 }
