@@ -5,6 +5,8 @@
 // straight from global memory and the last stage stores straight to global memory, so LDS is touched only between
 // stages: once for N = 16*16, twice for 8*8*8.
 #pragma once
+#include <type_traits>
+
 #include "nsl_device.hpp"
 
 namespace nsl {
@@ -179,13 +181,31 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, const double2 *
 #pragma unroll
       for (int j = 0; j < R; j++) {
         const int n = o0 + j * S;
-        if (LAST) store(n, t, v[k][j]);
-        else lds[LY::at(n, t)] = v[k][j];
+        if constexpr (LAST) {
+          // a store functor may also take the register slot (k, j): outputs can stay in registers
+          if constexpr (std::is_invocable<ST, int, int, double2, int, int>::value) store(n, t, v[k][j], k, j);
+          else store(n, t, v[k][j]);
+        } else {
+          lds[LY::at(n, t)] = v[k][j];
+        }
       }
     }
   }
   if (!LAST) __syncthreads();
 }
+
+// Register footprint of the last stage: outputs land in slots (k, j) at element n = bb + j*(N/R), line t, where
+// idx = threadIdx.x + k*NT, t = idx % T, bb = idx / T (Interleaved layout).
+template <class L> struct LastR;
+template <int R0> struct LastR<RList<R0>> { static constexpr int R = R0; };
+template <int R0, int R1, int... Rs> struct LastR<RList<R0, R1, Rs...>> { static constexpr int R = LastR<RList<R1, Rs...>>::R; };
+template <int N, int T, int NT> struct OutRegs {
+  static constexpr int R = LastR<typename Sched<N>::L>::R, S = N / R, TOT = S * T, K = (TOT + NT - 1) / NT;
+  double2 v[K][R];
+  static __device__ __forceinline__ bool valid(int k) { return TOT % NT == 0 || (int)threadIdx.x + k * NT < TOT; }
+  static __device__ __forceinline__ int line(int k) { return ((int)threadIdx.x + k * NT) % T; }
+  static __device__ __forceinline__ int elem(int k, int j) { return ((int)threadIdx.x + k * NT) / T + j * S; }
+};
 
 template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, class LD, class ST, int R0,
           int... Rs>

@@ -10,10 +10,18 @@
 #ifndef NSL_PERSISTENT
 #define NSL_PERSISTENT 0
 #endif
+#ifndef NSL_ZINV_UW_MAXN
+#define NSL_ZINV_UW_MAXN 768
+#endif
 #ifndef NSL_LP_ELEMS
 #define NSL_LP_ELEMS 4096
 #endif
 static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
+// x pass in row form (one row per workgroup, three complex lines per row) from N = 512 up: half the LDS of the
+// row-pair form, so twice the workgroups per CU
+#ifndef NSL_XNL_ROWS
+#define NSL_XNL_ROWS 1
+#endif
 static constexpr int fast_LP(int N) {
   return 6 * N * 8 <= NSL_LP_ELEMS ? 8 : 6 * N * 4 <= NSL_LP_ELEMS ? 4 : 6 * N * 2 <= NSL_LP_ELEMS ? 2 : 1;
 }
@@ -64,7 +72,7 @@ static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int no
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_zinv<N, T, MODE>, lds);
-  hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+  hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by)), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
 }
 
 template <int N>
@@ -73,6 +81,12 @@ static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, 
   const int lds = N * T * (int)sizeof(double2);
   const long long total = (long long)nout * fast_ntile(x) * x.g.by;
   static int c0 = 0, c1 = 0, c2 = 0;
+  if (mode == 1 && nout == 6 && N <= NSL_ZINV_UW_MAXN) {   // the rhs: u and omega from five transforms per tile
+    allow_lds(fast::kf_zinv_uw<N, T>, lds);
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by)), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
   if (!NSL_PERSISTENT) {
     if (mode == 0) kf_zinv_launch<N, 0>(x, U, X1, nout, gc, ctl);
     else if (mode == 1) kf_zinv_launch<N, 1>(x, U, X1, nout, gc, ctl);
@@ -97,7 +111,7 @@ static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp
   static int c = 0;
   if (!NSL_PERSISTENT) {
     allow_lds(fast::kf_yinv<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X1, X2, x.g, x.twN, ctl);
+    hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X1, X2, x.g, x.twN, ctl);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -114,7 +128,7 @@ static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp
   static int c = 0;
   if (!NSL_PERSISTENT) {
     allow_lds(fast::kf_yfwd<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X3, X4, x.g, x.twN, ctl);
+    hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X3, X4, x.g, x.twN, ctl);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -132,7 +146,7 @@ static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncom
   static int c = 0;
   if (!NSL_PERSISTENT) {
     allow_lds(fast::kf_zfwd<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_zfwd<N, T>), dim3((unsigned)(fast_ntile(x) * x.g.by), (unsigned)ncomp), dim3(NSL_NT), lds, x.st, X4, OUT, x.g, x.twN, inv, ctl);
+    hipLaunchKernelGGL((fast::kf_zfwd<N, T>), dim3((unsigned)(fast_ntile(x) * x.g.by), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X4, OUT, x.g, x.twN, inv, ctl);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -144,6 +158,19 @@ static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncom
 }
 template <int N>
 static int fast_xnl_n(const Xform &x, double2 *X2, Ctl *ctl) {
+#if NSL_XNL_ROWS
+  if constexpr (N >= 512) {
+    constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
+    const int lds = N * 3 * RP * (int)sizeof(double2);
+    allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);
+    const int nblk = (N * N + RP - 1) / RP;
+    hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk), dim3(NT), lds, x.st, X2, x.g, x.twN, x.pmax, ctl);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512)), dim3(NSL_NT), 0, x.st, (const double *)x.pmax, nblk, ctl);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+#endif
   constexpr int LP = fast_LP(N);
   const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
   const long long npairs = (long long)N * (N / 2);

@@ -20,11 +20,18 @@ __host__ __device__ constexpr Geo trunc_geo() {
   return g;
 }
 
+// Threads per workgroup of the pencil passes (P1, P2, P4, P5). With 8 lines of N >= 512 per workgroup, 512 threads
+// give each thread one butterfly per stage, which halves the registers per thread and doubles the waves per CU.
+#ifndef NSL_PNT_LARGE
+#define NSL_PNT_LARGE 512
+#endif
+template <int N> struct PassNT { static constexpr int v = N >= 512 ? NSL_PNT_LARGE : NSL_NT; };
+
 // P1: X[o][z][cy][kx] = z-inverse of field o (MODE 0 stored, 1 u/omega, 2 gradient of gc). Grid ntile*nout*by,
 // tile fastest: neighbouring workgroups write neighbouring segments of the same rows, and the nout workgroups that
 // read the same state tile run close together (their re-reads of the state hit L2).
 template <int N, int T, int MODE>
-__global__ __launch_bounds__(NSL_NT) void kf_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+__global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
                                                    const double2 *__restrict__ tw, int nout, int gc,
                                                    const Ctl *__restrict__ ctl) {
   if (ctl != nullptr && ctl->active == 0) return;
@@ -46,12 +53,83 @@ __global__ __launch_bounds__(NSL_NT) void kf_zinv(const double2 *__restrict__ U,
   auto store = [&](int n, int t, double2 v) {
     if (kx0 + t < g.hx) Xb[(size_t)n * plane + t] = v;
   };
-  fft<N, T, NSL_NT, 1, false>(lds, tw, load, store);
+  fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
+}
+
+// P1 for the rhs (u and omega), two workgroups per (kx tile, ky row). The curl is formed after the z transform,
+// in (kx, ky, z) space, from A_c = IFFT_z(u_c) and B_c = IFFT_z(i kz u_c): omega-hat = i k x u-hat (nslab.js:203-212)
+// is linear, so IFFT_z(omega-hat_x) = i ky A_z - B_y, IFFT_z(omega-hat_y) = B_x - i kx A_z,
+// IFFT_z(omega-hat_z) = i (kx A_y - ky A_x). Type 0 transforms u_x, u_y and writes u_x, u_y, omega_z; type 1
+// transforms u_z, i kz u_y, i kz u_x and writes u_z, omega_x, omega_y: five transforms and five state-tile reads per
+// tile instead of six and nine. Grid ntile * 2 * by, XCD-remapped so the pair shares an L2.
+template <int N, int T>
+__global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+                                                      const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+  if (ctl != nullptr && ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
+  using OR = OutRegs<N, T, PassNT<N>::v>;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = bid % ntile, rest = bid / ntile, type = rest & 1, cy = rest >> 1, kx0 = tile * T;
+  const double ky = kwv(full_of(cy, g), N);
+  const size_t ubase = (size_t)cy * g.hp + kx0;
+  double2 *Xb = X + (size_t)cy * g.hp + kx0;
+  OR A, B;
+  auto keepA = [&](int, int, double2 v, int k, int j) { A.v[k][j] = v; };
+  auto keepB = [&](int, int, double2 v, int k, int j) { B.v[k][j] = v; };
+  auto load_c = [&](int c, bool ikz) {
+    return [=](int n, int t) -> double2 {
+      const int cz = compact_of(n, g);
+      if (cz < 0 || kx0 + t >= g.hx) return make_double2(0.0, 0.0);
+      const double2 u = U[(size_t)c * g.D + ubase + (size_t)cz * plane + t];
+      if (!ikz) return u;
+      const double kz = kwv(n, N);
+      return make_double2(-kz * u.y, kz * u.x);
+    };
+  };
+  // out(o, f): write f(k, j) to field o at every register slot of this thread
+  auto out = [&](int o, auto f) {
+#pragma unroll
+    for (int k = 0; k < OR::K; k++) {
+      if (!OR::valid(k) || kx0 + OR::line(k) >= g.hx) continue;
+#pragma unroll
+      for (int j = 0; j < OR::R; j++) Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)] = f(k, j);
+    }
+  };
+  if (type == 0) {
+    auto lx = load_c(0, false);
+    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lx, keepA);
+    out(0, [&](int k, int j) { return A.v[k][j]; });
+    auto ly = load_c(1, false);
+    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, ly, keepB);
+    out(1, [&](int k, int j) { return B.v[k][j]; });
+    out(5, [&](int k, int j) {
+      const double kx = (double)(kx0 + OR::line(k));
+      const double pr = kx * B.v[k][j].x - ky * A.v[k][j].x, pi = kx * B.v[k][j].y - ky * A.v[k][j].y;
+      return make_double2(-pi, pr);
+    });
+  } else {
+    auto lz = load_c(2, false);
+    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lz, keepA);
+    out(2, [&](int k, int j) { return A.v[k][j]; });
+    auto ly = load_c(1, true);
+    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, ly, keepB);
+    out(3, [&](int k, int j) {
+      return make_double2(-ky * A.v[k][j].y - B.v[k][j].x, ky * A.v[k][j].x - B.v[k][j].y);
+    });
+    auto lx = load_c(0, true);
+    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lx, keepB);
+    out(4, [&](int k, int j) {
+      const double kx = (double)(kx0 + OR::line(k));
+      return make_double2(B.v[k][j].x + kx * A.v[k][j].y, B.v[k][j].y - kx * A.v[k][j].x);
+    });
+  }
 }
 
 // P2: X1[o][z][cy][kx] -> X2[o][z][y][kx]. Grid (ntile*N, ncomp).
 template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kf_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
+__global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
                                                    const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
@@ -67,12 +145,12 @@ __global__ __launch_bounds__(NSL_NT) void kf_yinv(const double2 *__restrict__ X1
   auto store = [&](int n, int t, double2 v) {
     if (kx0 + t < g.hx) Ob[n * g.hp + t] = v;
   };
-  fft<N, T, NSL_NT, 1, false>(lds, tw, load, store);
+  fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
 }
 
 // P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
 template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
+__global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
                                                    const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
@@ -88,12 +166,12 @@ __global__ __launch_bounds__(NSL_NT) void kf_yfwd(const double2 *__restrict__ X3
     const int cy = compact_of(n, g);
     if (cy >= 0 && kx0 + t < g.hx) Ob[cy * g.hp + t] = v;
   };
-  fft<N, T, NSL_NT, -1, false>(lds, tw, load, store);
+  fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
 
 // P5: X4[c][z][cy][kx] -> OUT[c][cz][cy][kx] * inv (retained kz). Grid (ntile*by, ncomp).
 template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kf_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
+__global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
                                                    const double2 *__restrict__ tw, double inv, const Ctl *__restrict__ ctl) {
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
@@ -110,7 +188,7 @@ __global__ __launch_bounds__(NSL_NT) void kf_zfwd(const double2 *__restrict__ X4
     const int cz = compact_of(n, g);
     if (cz >= 0 && kx0 + t < g.hx) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
   };
-  fft<N, T, NSL_NT, -1, false>(lds, tw, load, store);
+  fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
 
 // LDS layout of the x pass (interleaved measured faster than line-major on gfx950).
@@ -160,11 +238,6 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   const int grp = blockIdx.x;
   load_group(grp);
   {
-  // modes hx .. N-hx of every line are zero
-  for (int e = threadIdx.x; e < T * (N - 2 * hx + 1); e += NSL_NT) {
-    const int t = e / (N - 2 * hx + 1), n = hx + (e - t * (N - 2 * hx + 1));
-    lds[L1::at(n, t)] = make_double2(0.0, 0.0);
-  }
 #pragma unroll
   for (int i = 0; i < KL; i++) {
     const int e = threadIdx.x + i * NSL_NT;
@@ -175,7 +248,10 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
     }
   }
   __syncthreads();
-  auto ldl = [&](int n, int t) -> double2 { return lds[L1::at(n, t)]; };
+  // modes hx .. N-hx of every line are zero: never written, never read
+  auto ldl = [&](int n, int t) -> double2 {
+    return (n >= hx && n <= N - hx) ? make_double2(0.0, 0.0) : lds[L1::at(n, t)];
+  };
   auto stl = [&](int n, int t, double2 v) { lds[L1::at(n, t)] = v; };
   fft<N, T, NSL_NT, 1, true, L1>(lds, tw, ldl, stl);
   __syncthreads();
@@ -215,7 +291,10 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   }
   __syncthreads();
   auto ldl2 = [&](int n, int t) -> double2 { return lds[L2::at(n, t)]; };
-  auto stl2 = [&](int n, int t, double2 v) { lds[L2::at(n, t)] = v; };
+  // only modes k < hx and their mirrors N-k are unpacked
+  auto stl2 = [&](int n, int t, double2 v) {
+    if (n < hx || n > N - hx) lds[L2::at(n, t)] = v;
+  };
   fft<N, T2, NSL_NT, -1, true, L2>(lds, tw, ldl2, stl2);
   __syncthreads();
   constexpr int ITEMS2 = T2 * hx, KS = (ITEMS2 + NSL_NT - 1) / NSL_NT;
@@ -237,6 +316,118 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   }
   }
   block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds);
+}
+
+// P3 with RP rows per workgroup of NT threads (half the LDS of the row-pair form for RP = 1). Each row's six
+// half-spectra ride as three complex lines, (u_x + i u_y), (u_z + i om_x), (om_y + i om_z); after the inverse the
+// product u x omega is formed per point, and (n_x + i n_y), (n_z + i 0) go through the forward transform.
+template <int N, int RP, int NT>
+__global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl) {
+  if (ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int T = 3 * RP, T2 = 2 * RP, nrows = N * N;
+  using L1 = Interleaved<N, T>;
+  using L2 = Interleaved<N, T2>;
+  constexpr int hx = g.hx, HP = g.hp;
+  constexpr int ITEMS = T * hx, KL = (ITEMS + NT - 1) / NT;
+  constexpr size_t cs = (size_t)N * N * HP;
+  const int row0 = blockIdx.x * RP;
+  double2 A[KL], B[KL];
+#pragma unroll
+  for (int i = 0; i < KL; i++) {
+    const int e = threadIdx.x + i * NT;
+    A[i] = B[i] = make_double2(0.0, 0.0);
+    if (ITEMS % NT == 0 || e < ITEMS) {
+      const int t = e / hx, k = e - t * hx, r = t / 3, l = t - r * 3;   // line l of row r: fields 2l, 2l+1
+      if (row0 + r < nrows) {
+        const size_t base = (size_t)(row0 + r) * HP + k;
+        A[i] = X2[2 * l * cs + base];
+        B[i] = X2[(2 * l + 1) * cs + base];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < KL; i++) {
+    const int e = threadIdx.x + i * NT;
+    if (ITEMS % NT == 0 || e < ITEMS) {
+      const int t = e / hx, k = e - t * hx;
+      lds[L1::at(k, t)] = make_double2(A[i].x - B[i].y, A[i].y + B[i].x);
+      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = make_double2(A[i].x + B[i].y, B[i].x - A[i].y);
+    }
+  }
+  __syncthreads();
+  auto ldl = [&](int n, int t) -> double2 {
+    return (n >= hx && n <= N - hx) ? make_double2(0.0, 0.0) : lds[L1::at(n, t)];
+  };
+  auto stl = [&](int n, int t, double2 v) { lds[L1::at(n, t)] = v; };
+  fft<N, T, NT, 1, true, L1>(lds, tw, ldl, stl);
+  __syncthreads();
+  constexpr int KI = (N * RP + NT - 1) / NT;
+  double nv[KI][3];
+  double om2 = 0.0, u2 = 0.0;
+#pragma unroll
+  for (int k2 = 0; k2 < KI; k2++) {
+    const int it = threadIdx.x + k2 * NT;
+    if ((N * RP) % NT == 0 || it < N * RP) {
+      const int x = it % N, r = it / N;
+      const double2 Z0 = lds[L1::at(x, 3 * r)], Z1 = lds[L1::at(x, 3 * r + 1)], Z2 = lds[L1::at(x, 3 * r + 2)];
+      const double a = Z0.x, b = Z0.y, c = Z1.x, pp = Z1.y, q = Z2.x, rr = Z2.y;   // u = (a, b, c), omega = (pp, q, rr)
+      const double o2 = pp * pp + q * q + rr * rr, uu = a * a + b * b + c * c;
+      om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+      nv[k2][0] = b * rr - c * q; nv[k2][1] = c * pp - a * rr; nv[k2][2] = a * q - b * pp;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < KI; k2++) {
+    const int it = threadIdx.x + k2 * NT;
+    if ((N * RP) % NT == 0 || it < N * RP) {
+      const int x = it % N, r = it / N;
+      lds[L2::at(x, 2 * r)] = make_double2(nv[k2][0], nv[k2][1]);
+      lds[L2::at(x, 2 * r + 1)] = make_double2(nv[k2][2], 0.0);
+    }
+  }
+  __syncthreads();
+  auto ldl2 = [&](int n, int t) -> double2 { return lds[L2::at(n, t)]; };
+  auto stl2 = [&](int n, int t, double2 v) {
+    if (n < hx || n > N - hx) lds[L2::at(n, t)] = v;
+  };
+  fft<N, T2, NT, -1, true, L2>(lds, tw, ldl2, stl2);
+  __syncthreads();
+  constexpr int ITEMS2 = RP * hx, KS = (ITEMS2 + NT - 1) / NT;
+#pragma unroll
+  for (int i = 0; i < KS; i++) {
+    const int e = threadIdx.x + i * NT;
+    if (!(ITEMS2 % NT == 0 || e < ITEMS2)) continue;
+    const int r = e / hx, k = e - r * hx;
+    if (row0 + r >= nrows) continue;
+    const size_t base = (size_t)(row0 + r) * HP + k;
+    const int kk = (N - k) % N;
+    {  // n_x + i n_y
+      const double2 Zk = lds[L2::at(k, 2 * r)], Zc = lds[L2::at(kk, 2 * r)];
+      const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+      X2[base] = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+      X2[cs + base] = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+    }
+    {  // n_z (imaginary part zero)
+      const double2 Zk = lds[L2::at(k, 2 * r + 1)], Zc = lds[L2::at(kk, 2 * r + 1)];
+      X2[2 * cs + base] = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+    }
+  }
+  // per-workgroup maxima
+  om2 = wave_max(om2); u2 = wave_max(u2);
+  __syncthreads();
+  double *scratch = (double *)lds;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { scratch[2 * wid] = om2; scratch[2 * wid + 1] = u2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ma = scratch[0], mb = scratch[1];
+    for (int w = 1; w < NT / 64; w++) { ma = fmax(ma, scratch[2 * w]); mb = fmax(mb, scratch[2 * w + 1]); }
+    pmax[2 * (size_t)blockIdx.x] = ma; pmax[2 * (size_t)blockIdx.x + 1] = mb;
+  }
 }
 
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred

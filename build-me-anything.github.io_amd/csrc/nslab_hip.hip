@@ -273,8 +273,8 @@ static int xform_init(Xform &x, const Geo &g, hipStream_t st) {
     }
     HIP_TRY(hipMalloc(&x.twN, g.N * sizeof(double2)));
     HIP_TRY(hipMemcpy(x.twN, tw.data(), g.N * sizeof(double2), hipMemcpyHostToDevice));
-    const size_t nblk = ((size_t)g.N * (g.N / 2) + x.fLP - 1) / x.fLP;
-    HIP_TRY(hipMalloc(&x.pmax, 2 * nblk * sizeof(double)));
+    // one (max |omega|^2, max |u|^2) slot per x-pass workgroup: at most one workgroup per x row
+    HIP_TRY(hipMalloc(&x.pmax, 2 * (size_t)g.N * g.N * sizeof(double)));
   }
   return NSL_OK;
 }

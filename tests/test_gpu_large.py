@@ -1,0 +1,106 @@
+"""Full-size parity of the size-specialised kernels (N = 192 .. 768) through a resolution-independence property.
+
+A velocity field whose modes all satisfy |k_i| <= 2 stays band-limited through one RK4 step with a fixed dt: each
+rhs evaluation doubles the largest wavenumber (u x omega of two fields with |k_i| <= m has |k_i| <= 2m), so the four
+stages reach |k_i| <= 16 and the new state holds modes |k_i| <= 16 only. No product aliases on a grid with N >= 48,
+and the 2/3 mask (|k_i| <= N/3) removes nothing. In exact arithmetic every N >= 48 therefore gives the same
+coefficients; the oracle at N = 48 is the reference, and the large grids must agree with it to roundoff. The field is
+random (no symmetry hides a transposed axis or a sign), divergence-free and Hermitian-consistent.
+Tolerance: 1e-13 relative (L-infinity over the retained modes), modes |k_i| > 16 below 1e-13 of the largest mode."""
+import numpy as np
+import pytest
+
+import capi
+from conftest import rel_inf
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KB = 2          # band limit of the initial field
+KOUT = 16       # band limit after one step
+DT = 0.005
+RE = 1600.0
+
+
+def band_limited_modes(seed):
+    """Divergence-free random modes with |k_i| <= KB: dict (kz, ky, kx) -> complex[3], kx >= 0, Hermitian on kx = 0."""
+    rng = np.random.default_rng(seed)
+    n0 = 8
+    F = np.zeros((3, n0, n0, n0 // 2 + 1), complex)
+    r = range(-KB, KB + 1)
+    for c in range(3):
+        for kz in r:
+            for ky in r:
+                for kx in range(0, KB + 1):
+                    F[c, kz % n0, ky % n0, kx] = complex(rng.standard_normal(), rng.standard_normal())
+    # through a real field, so the kx = 0 plane is Hermitian-consistent
+    f = np.fft.irfftn(F, s=(n0, n0, n0), axes=(1, 2, 3))
+    G = np.fft.rfftn(f, axes=(1, 2, 3)) / n0 ** 3
+    modes = {}
+    for kz in r:
+        for ky in r:
+            for kx in range(0, KB + 1):
+                v = G[:, kz % n0, ky % n0, kx].copy()
+                k = np.array([kx, ky, kz], float)
+                k2 = k @ k
+                if k2 == 0:
+                    continue
+                v -= k * (k @ v) / k2
+                modes[(kz, ky, kx)] = v
+    return modes
+
+
+def state_of(N, modes):
+    NH = N // 2 + 1
+    S = np.zeros((6, N * N * NH))
+    for (kz, ky, kx), v in modes.items():
+        idx = ((kz % N) * N + (ky % N)) * NH + kx
+        for c in range(3):
+            S[2 * c, idx] = v[c].real
+            S[2 * c + 1, idx] = v[c].imag
+    return S
+
+
+def low_block(N, S, K):
+    """Modes |kz|, |ky| <= K, 0 <= kx <= K of a state in the reference layout, as (6, 2K+1, 2K+1, K+1)."""
+    NH = N // 2 + 1
+    A = S.reshape(6, N, N, NH)
+    zi = np.array([k % N for k in range(-K, K + 1)])
+    return A[:, zi][:, :, zi][:, :, :, :K + 1]
+
+
+@pytest.fixture(scope="module")
+def reference():
+    modes = band_limited_modes(11)
+    o = O.OracleSolver(N=48, Re=RE, ic="tgv", dt=DT)
+    o.set_state(state_of(48, modes))
+    assert o.run(1) == 1
+    return modes, low_block(48, o.state(), KOUT), o.last
+
+
+@pytest.mark.parametrize("N", [192, 256, 384, 512, 768])
+def test_band_limited_step_matches_oracle_at_48(reference, N):
+    modes, ref, ref_last = reference
+    s = capi.Solver(N=N, Re=RE, ic="tgv", dt=DT)
+    s.set_state(state_of(N, modes))
+    assert s.run(1) == 1
+    S = s.state()
+    last = s.last
+    s.close()
+    got = low_block(N, S, KOUT)
+    assert rel_inf(got, ref) < 1e-13, N
+    scale = np.max(np.abs(ref))
+    A = np.abs(S.reshape(6, N, N, N // 2 + 1))
+    del S
+    A[low_mask(N)] = 0.0
+    assert np.max(A) < 1e-13 * scale, N
+    del A
+    # spectral sums of the new state (the fifth rhs): E, Z, P and eps = 2 nu Z
+    for k in ("E", "Z", "P", "eps"):
+        assert abs(last[k] - ref_last[k]) <= 1e-13 * abs(ref_last[k]), (N, k)
+
+
+def low_mask(N):
+    """Index expression selecting the modes |kz|, |ky|, kx <= KOUT of a (6, N, N, NH) view."""
+    zi = np.array([k % N for k in range(-KOUT, KOUT + 1)])
+    return (slice(None), zi[:, None, None], zi[None, :, None], np.arange(KOUT + 1)[None, None, :])
