@@ -139,6 +139,15 @@ __device__ void lds_fft(double2 *__restrict__ buf, int T, const DPlan &P, double
   }
 }
 
+// ------------------------------------------------------------------------------------------------ slots
+// A pool holds several solves of one N, slot z's buffers sb bytes after slot z-1's. A pass launched with
+// gridDim.z = slots runs every slot at once, each workgroup on the buffers of slot blockIdx.z; sb = 0 with one slot
+// is the single-solve case.
+template <class T>
+__device__ __forceinline__ T *at_slot(T *p, size_t sb) {
+  return p == nullptr ? p : reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * sb);
+}
+
 // ------------------------------------------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll

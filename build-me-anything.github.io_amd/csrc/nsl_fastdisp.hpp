@@ -7,9 +7,6 @@
 #ifndef NSL_T_SMALL
 #define NSL_T_SMALL 8
 #endif
-#ifndef NSL_PERSISTENT
-#define NSL_PERSISTENT 0
-#endif
 #ifndef NSL_ZINV_UW_MAXN
 #define NSL_ZINV_UW_MAXN 768
 #endif
@@ -45,140 +42,81 @@ static bool fast_setup(Xform &x) {
 
 static int fast_ntile(const Xform &x) { return (x.g.hx + x.fT - 1) / x.fT; }
 
-static int device_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    hipDeviceProp_t p;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
-    if (cus <= 0) cus = 256;
-  }
-  return cus;
-}
-
-// Persistent grid: every CU holds as many workgroups as its LDS and registers allow, and they stride the tiles.
-template <typename K>
-static unsigned persistent_grid(K kernel, int lds, long long total, int &cache) {
-  if (!cache) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NSL_NT, lds) != hipSuccess || nb < 1) nb = 1;
-    cache = nb * device_cus();
-  }
-  return (unsigned)std::max<long long>(1, std::min<long long>(total, cache));
-}
-
 template <int N, int MODE>
-static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int nout, int gc, const Ctl *ctl) {
+static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int nout, int gc, const Ctl *ctl, const Batch &bt) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_zinv<N, T, MODE>, lds);
-  hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by)), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
+  hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by), 1, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl, bt.sb);
 }
 
 template <int N>
-static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
+static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
-  const long long total = (long long)nout * fast_ntile(x) * x.g.by;
-  static int c0 = 0, c1 = 0, c2 = 0;
   if (mode == 1 && nout == 6 && N <= NSL_ZINV_UW_MAXN) {   // the rhs: u and omega from five transforms per tile
     allow_lds(fast::kf_zinv_uw<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by)), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl);
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by), 1, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl, bt.sb);
     LAUNCH_CHECK();
     return NSL_OK;
   }
-  if (!NSL_PERSISTENT) {
-    if (mode == 0) kf_zinv_launch<N, 0>(x, U, X1, nout, gc, ctl);
-    else if (mode == 1) kf_zinv_launch<N, 1>(x, U, X1, nout, gc, ctl);
-    else kf_zinv_launch<N, 2>(x, U, X1, nout, gc, ctl);
-  } else if (mode == 0) {
-    allow_lds(fast::kp_zinv<N, T, 0>, lds);
-    hipLaunchKernelGGL((fast::kp_zinv<N, T, 0>), dim3(persistent_grid(fast::kp_zinv<N, T, 0>, lds, total, c0)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
-  } else if (mode == 1) {
-    allow_lds(fast::kp_zinv<N, T, 1>, lds);
-    hipLaunchKernelGGL((fast::kp_zinv<N, T, 1>), dim3(persistent_grid(fast::kp_zinv<N, T, 1>, lds, total, c1)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
-  } else {
-    allow_lds(fast::kp_zinv<N, T, 2>, lds);
-    hipLaunchKernelGGL((fast::kp_zinv<N, T, 2>), dim3(persistent_grid(fast::kp_zinv<N, T, 2>, lds, total, c2)), dim3(NSL_NT), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl);
-  }
+  if (mode == 0) kf_zinv_launch<N, 0>(x, U, X1, nout, gc, ctl, bt);
+  else if (mode == 1) kf_zinv_launch<N, 1>(x, U, X1, nout, gc, ctl, bt);
+  else kf_zinv_launch<N, 2>(x, U, X1, nout, gc, ctl, bt);
   LAUNCH_CHECK();
   return NSL_OK;
 }
 template <int N>
-static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
+static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
-  static int c = 0;
-  if (!NSL_PERSISTENT) {
-    allow_lds(fast::kf_yinv<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X1, X2, x.g, x.twN, ctl);
-    LAUNCH_CHECK();
-    return NSL_OK;
-  }
-  allow_lds(fast::kp_yinv<N, T>, lds);
-  const unsigned grid = persistent_grid(fast::kp_yinv<N, T>, lds, (long long)fast_ntile(x) * N * ncomp, c);
-  hipLaunchKernelGGL((fast::kp_yinv<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X1, X2, x.g, x.twN, ncomp, ctl);
+  allow_lds(fast::kf_yinv<N, T>, lds);
+  hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X1, X2, x.g, x.twN, ctl, bt.sb);
   LAUNCH_CHECK();
   return NSL_OK;
 }
 template <int N>
-static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
+static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
-  static int c = 0;
-  if (!NSL_PERSISTENT) {
-    allow_lds(fast::kf_yfwd<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X3, X4, x.g, x.twN, ctl);
-    LAUNCH_CHECK();
-    return NSL_OK;
-  }
-  allow_lds(fast::kp_yfwd<N, T>, lds);
-  const unsigned grid = persistent_grid(fast::kp_yfwd<N, T>, lds, (long long)fast_ntile(x) * N * ncomp, c);
-  hipLaunchKernelGGL((fast::kp_yfwd<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X3, X4, x.g, x.twN, ncomp, ctl);
+  allow_lds(fast::kf_yfwd<N, T>, lds);
+  hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X3, X4, x.g, x.twN, ctl, bt.sb);
   LAUNCH_CHECK();
   return NSL_OK;
 }
 template <int N>
-static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
+static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   const double inv = 1.0 / ((double)N * N * N);
-  static int c = 0;
-  if (!NSL_PERSISTENT) {
-    allow_lds(fast::kf_zfwd<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_zfwd<N, T>), dim3((unsigned)(fast_ntile(x) * x.g.by), (unsigned)ncomp), dim3(fast::PassNT<N>::v), lds, x.st, X4, OUT, x.g, x.twN, inv, ctl);
-    LAUNCH_CHECK();
-    return NSL_OK;
-  }
-  allow_lds(fast::kp_zfwd<N, T>, lds);
-  const unsigned grid = persistent_grid(fast::kp_zfwd<N, T>, lds, (long long)fast_ntile(x) * x.g.by * ncomp, c);
-  hipLaunchKernelGGL((fast::kp_zfwd<N, T>), dim3(grid), dim3(NSL_NT), lds, x.st, X4, OUT, x.g, x.twN, inv, ncomp, ctl);
+  allow_lds(fast::kf_zfwd<N, T>, lds);
+  hipLaunchKernelGGL((fast::kf_zfwd<N, T>), dim3((unsigned)(fast_ntile(x) * x.g.by), (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X4, OUT, x.g, x.twN, inv, ctl, bt.sb);
   LAUNCH_CHECK();
   return NSL_OK;
 }
+// pmax: 2 * (workgroups of the x pass) doubles per slot, at most 2 N^2
 template <int N>
-static int fast_xnl_n(const Xform &x, double2 *X2, Ctl *ctl) {
+static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
+  int nblk;
 #if NSL_XNL_ROWS
   if constexpr (N >= 512) {
     constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
     const int lds = N * 3 * RP * (int)sizeof(double2);
     allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);
-    const int nblk = (N * N + RP - 1) / RP;
-    hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk), dim3(NT), lds, x.st, X2, x.g, x.twN, x.pmax, ctl);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512)), dim3(NSL_NT), 0, x.st, (const double *)x.pmax, nblk, ctl);
-    LAUNCH_CHECK();
-    return NSL_OK;
-  }
+    nblk = (N * N + RP - 1) / RP;
+    hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk, 1, bt.nz), dim3(NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+  } else
 #endif
-  constexpr int LP = fast_LP(N);
-  const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
-  const long long npairs = (long long)N * (N / 2);
-  allow_lds(fast::kf_xnl<N, LP>, lds);
-  const int nblk = (int)((npairs + LP - 1) / LP);
-  hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, x.pmax, ctl);
+  {
+    constexpr int LP = fast_LP(N);
+    const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
+    const long long npairs = (long long)N * (N / 2);
+    allow_lds(fast::kf_xnl<N, LP>, lds);
+    nblk = (int)((npairs + LP - 1) / LP);
+    hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk, 1, bt.nz), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+  }
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512)), dim3(NSL_NT), 0, x.st, (const double *)x.pmax, nblk, ctl);
+  hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512), 1, bt.nz), dim3(NSL_NT), 0, x.st, (const double *)pmax, nblk, ctl, bt.sb);
   LAUNCH_CHECK();
   return NSL_OK;
 }
@@ -201,18 +139,18 @@ static int fast_xnl_n(const Xform &x, double2 *X2, Ctl *ctl) {
     default: return fail(NSL_ERR_STATE, "no size-specialised kernels for this N"); \
   }
 
-static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
-  NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl)
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl, bt)
 }
-static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
-  NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl)
+static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl, bt)
 }
-static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
-  NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl)
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl, bt)
 }
-static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
-  NSL_DISPATCH(fast_zfwd_n, X4, OUT, ncomp, ctl)
+static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_zfwd_n, X4, OUT, ncomp, ctl, bt)
 }
-static int fast_dispatch_xnl(const Xform &x, double2 *X2, Ctl *ctl) {
-  NSL_DISPATCH(fast_xnl_n, X2, ctl)
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt)
 }

@@ -33,7 +33,8 @@ template <int N> struct PassNT { static constexpr int v = N >= 512 ? NSL_PNT_LAR
 template <int N, int T, int MODE>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
                                                    const double2 *__restrict__ tw, int nout, int gc,
-                                                   const Ctl *__restrict__ ctl) {
+                                                   const Ctl *__restrict__ ctl, size_t sb) {
+  U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -64,7 +65,8 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restric
 // tile instead of six and nine. Grid ntile * 2 * by, XCD-remapped so the pair shares an L2.
 template <int N, int T>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
-                                                      const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+                                                      const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+  U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -130,7 +132,8 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__rest
 // P2: X1[o][z][cy][kx] -> X2[o][z][y][kx]. Grid (ntile*N, ncomp).
 template <int N, int T>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
-                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+  X1 = at_slot(X1, sb); X2 = at_slot(X2, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -151,7 +154,8 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
 // P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
 template <int N, int T>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
-                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl) {
+                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+  X3 = at_slot(X3, sb); X4 = at_slot(X4, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -172,7 +176,9 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restric
 // P5: X4[c][z][cy][kx] -> OUT[c][cz][cy][kx] * inv (retained kz). Grid (ntile*by, ncomp).
 template <int N, int T>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
-                                                   const double2 *__restrict__ tw, double inv, const Ctl *__restrict__ ctl) {
+                                                   const double2 *__restrict__ tw, double inv, const Ctl *__restrict__ ctl,
+                                                   size_t sb) {
+  X4 = at_slot(X4, sb); OUT = at_slot(OUT, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -201,7 +207,8 @@ using XnlLayout = typename std::conditional<NSL_XNL_LINEMAJOR != 0, LineMajor<N,
 // P3: in place on X2 (6 fields in, 3 out), LP row pairs per workgroup. Same packing and product as k_xnl.
 template <int N, int LP>
 __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
-                                                  double *__restrict__ pmax, Ctl *__restrict__ ctl) {
+                                                  double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -323,7 +330,8 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
 // product u x omega is formed per point, and (n_x + i n_y), (n_z + i 0) go through the forward transform.
 template <int N, int RP, int NT>
 __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
-                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl) {
+                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
@@ -432,8 +440,10 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
 
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
 // workgroups, one atomic pair each (the max is exact, so the order does not matter).
-__global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl) {
+__global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl,
+                                                        size_t sb) {
   __shared__ double scratch[8];
+  pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   double a = 0.0, b = 0.0;
   for (int i = blockIdx.x * NSL_NT + threadIdx.x; i < nblk; i += gridDim.x * NSL_NT) {
@@ -441,150 +451,6 @@ __global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict_
     b = fmax(b, pmax[2 * (size_t)i + 1]);
   }
   block_atomic_max2(a, b, &ctl->om2, &ctl->u2, scratch);
-}
-
-}  // namespace fast
-}  // namespace nsl
-
-// ------------------------------------------------------------------------------------------------ persistent forms
-// A grid of (CUs x resident workgroups) walks the tiles of a pass; each workgroup alternates two register sets so
-// the loads of its next tile are in flight while it transforms the current one.
-namespace nsl {
-namespace fast {
-
-template <int N, int T, class LoadT, class RunT>
-__device__ __forceinline__ void pipeline(int total, LoadT &loadTile, RunT &runTile) {
-  Regs<N, T, NSL_NT> A, B;
-  int it = blockIdx.x;
-  if (it >= total) return;
-  loadTile(A, it);
-  while (true) {
-    const int i1 = it + gridDim.x;
-    if (i1 < total) loadTile(B, i1);
-    runTile(A, it);
-    if (i1 >= total) break;
-    const int i2 = i1 + gridDim.x;
-    if (i2 < total) loadTile(A, i2);
-    runTile(B, i1);
-    if (i2 >= total) break;
-    it = i2;
-  }
-}
-
-// P1 persistent: tile index it = o + nout*(tile + ntile*cy).
-template <int N, int T, int MODE>
-__global__ __launch_bounds__(NSL_NT) void kp_zinv(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
-                                                   const double2 *__restrict__ tw, int nout, int gc,
-                                                   const Ctl *__restrict__ ctl) {
-  if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
-  constexpr Geo g = trunc_geo<N>();
-  const int ntile = (g.hx + T - 1) / T, total = nout * ntile * g.by, plane = g.by * g.hp;
-  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, rest = it / ntile, o = rest % nout, cy = rest / nout, kx0 = tile * T;
-    const double ky = kwv(full_of(cy, g), N);
-    const size_t ubase = (size_t)cy * g.hp + kx0;
-    auto load = [&](int n, int t) -> double2 {
-      const int kx = kx0 + t, cz = compact_of(n, g);
-      if (cz < 0 || kx >= g.hx) return make_double2(0.0, 0.0);
-      return zsrc<MODE>(U, g.D, ubase + (size_t)cz * plane + t, o, (double)kx, ky, kwv(n, N), gc);
-    };
-    load_first<N, T, NSL_NT>(r, load);
-  };
-  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, rest = it / ntile, o = rest % nout, cy = rest / nout, kx0 = tile * T;
-    double2 *Xb = X + ((size_t)o * N * g.by + cy) * g.hp + kx0;
-    auto store = [&](int n, int t, double2 v) {
-      if (kx0 + t < g.hx) Xb[(size_t)n * plane + t] = v;
-    };
-    fft_regs<N, T, NSL_NT, 1>(lds, tw, r, store);
-  };
-  pipeline<N, T>(total, loadTile, runTile);
-}
-
-// P2 persistent: it = tile + ntile*(z + N*o).
-template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kp_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
-                                                   const double2 *__restrict__ tw, int ncomp, const Ctl *__restrict__ ctl) {
-  if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
-  constexpr Geo g = trunc_geo<N>();
-  const int ntile = (g.hx + T - 1) / T, total = ntile * N * ncomp;
-  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, zo = it / ntile, kx0 = tile * T;
-    const double2 *Ib = X1 + (size_t)zo * g.by * g.hp + kx0;
-    auto load = [&](int n, int t) -> double2 {
-      const int cy = compact_of(n, g);
-      return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
-    };
-    load_first<N, T, NSL_NT>(r, load);
-  };
-  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, zo = it / ntile, kx0 = tile * T;
-    double2 *Ob = X2 + (size_t)zo * N * g.hp + kx0;
-    auto store = [&](int n, int t, double2 v) {
-      if (kx0 + t < g.hx) Ob[n * g.hp + t] = v;
-    };
-    fft_regs<N, T, NSL_NT, 1>(lds, tw, r, store);
-  };
-  pipeline<N, T>(total, loadTile, runTile);
-}
-
-// P4 persistent: it = tile + ntile*(z + N*c).
-template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kp_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
-                                                   const double2 *__restrict__ tw, int ncomp, const Ctl *__restrict__ ctl) {
-  if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
-  constexpr Geo g = trunc_geo<N>();
-  const int ntile = (g.hx + T - 1) / T, total = ntile * N * ncomp;
-  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, zc = it / ntile, kx0 = tile * T;
-    const double2 *Ib = X3 + (size_t)zc * N * g.hp + kx0;
-    auto load = [&](int n, int t) -> double2 {
-      return kx0 + t < g.hx ? Ib[n * g.hp + t] : make_double2(0.0, 0.0);
-    };
-    load_first<N, T, NSL_NT>(r, load);
-  };
-  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, zc = it / ntile, kx0 = tile * T;
-    double2 *Ob = X4 + (size_t)zc * g.by * g.hp + kx0;
-    auto store = [&](int n, int t, double2 v) {
-      const int cy = compact_of(n, g);
-      if (cy >= 0 && kx0 + t < g.hx) Ob[cy * g.hp + t] = v;
-    };
-    fft_regs<N, T, NSL_NT, -1>(lds, tw, r, store);
-  };
-  pipeline<N, T>(total, loadTile, runTile);
-}
-
-// P5 persistent: it = tile + ntile*(cy + by*c).
-template <int N, int T>
-__global__ __launch_bounds__(NSL_NT) void kp_zfwd(const double2 *__restrict__ X4, double2 *__restrict__ OUT, const Geo,
-                                                   const double2 *__restrict__ tw, double inv, int ncomp,
-                                                   const Ctl *__restrict__ ctl) {
-  if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
-  constexpr Geo g = trunc_geo<N>();
-  const int ntile = (g.hx + T - 1) / T, total = ntile * g.by * ncomp, plane = g.by * g.hp;
-  auto loadTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, rest = it / ntile, cy = rest % g.by, c = rest / g.by, kx0 = tile * T;
-    const double2 *Ib = X4 + (size_t)c * N * plane + (size_t)cy * g.hp + kx0;
-    auto load = [&](int n, int t) -> double2 {
-      return kx0 + t < g.hx ? Ib[(size_t)n * plane + t] : make_double2(0.0, 0.0);
-    };
-    load_first<N, T, NSL_NT>(r, load);
-  };
-  auto runTile = [&](Regs<N, T, NSL_NT> &r, int it) {
-    const int tile = it % ntile, rest = it / ntile, cy = rest % g.by, c = rest / g.by, kx0 = tile * T;
-    double2 *Ob = OUT + (size_t)c * g.D + (size_t)cy * g.hp + kx0;
-    auto store = [&](int n, int t, double2 v) {
-      const int cz = compact_of(n, g);
-      if (cz >= 0 && kx0 + t < g.hx) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
-    };
-    fft_regs<N, T, NSL_NT, -1>(lds, tw, r, store);
-  };
-  pipeline<N, T>(total, loadTile, runTile);
 }
 
 }  // namespace fast

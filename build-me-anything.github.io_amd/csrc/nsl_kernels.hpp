@@ -363,8 +363,10 @@ template <int STAGE>
 __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ NL, const double2 *U,
                                                     double2 *S, double2 *__restrict__ ACC,
                                                     double2 *TS, const Geo g,
-                                                    double *__restrict__ partials, Ctl *__restrict__ ctl) {
+                                                    double *__restrict__ partials, Ctl *__restrict__ ctl, size_t sb) {
   __shared__ double scratch[8 * NSL_NT];
+  NL = at_slot(NL, sb); U = at_slot(U, sb); S = at_slot(S, sb); ACC = at_slot(ACC, sb); TS = at_slot(TS, sb);
+  partials = at_slot(partials, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   const double nu = ctl->nu;
   const double dt = STAGE == 1 ? choose_dt(ctl, sqrt(__longlong_as_double((long long)ctl->u2))) : ctl->dt;
@@ -428,9 +430,10 @@ __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ N
 // in flight; mode 4: `last` (rhs of the current state) and, when rec != null, the end of the step: budgets
 // (nslab.js:326-329), t += dt, step++, one StepRecDev.
 __global__ __launch_bounds__(NSL_NT) void k_finalize(const double *__restrict__ partials, int nblk, int mode,
-                                                      StepRecDev *__restrict__ rec, Ctl *__restrict__ ctl) {
+                                                      StepRecDev *__restrict__ rec, Ctl *__restrict__ ctl, size_t sb) {
   __shared__ double scratch[8 * NSL_NT];
   __shared__ double tot[8];
+  partials = at_slot(partials, sb); rec = at_slot(rec, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nblk; b += NSL_NT)
@@ -466,7 +469,8 @@ __global__ __launch_bounds__(NSL_NT) void k_finalize(const double *__restrict__ 
 }
 
 // Start of a step: the stop predicate of run() (nslab.js:334) and dt of this step (nslab.js:314).
-__global__ void k_begin_step(Ctl *ctl) {
+__global__ void k_begin_step(Ctl *ctl, size_t sb) {
+  ctl = at_slot(ctl, sb);
   const bool act = ctl->nDone < ctl->nTarget && !(ctl->hasTEnd && ctl->t >= ctl->tEnd - 1e-12);
   ctl->active = act ? 1 : 0;
   if (act) {
@@ -474,8 +478,8 @@ __global__ void k_begin_step(Ctl *ctl) {
     for (int i = 0; i < 8; i++) ctl->diag[0][i] = ctl->diag[4][i];
   }
 }
-__global__ void k_set_active(Ctl *ctl, int v) { ctl->active = v; }
-__global__ void k_chunk_begin(Ctl *ctl) { ctl->nChunk = 0; }
+__global__ void k_set_active(Ctl *ctl, int v, size_t sb) { at_slot(ctl, sb)->active = v; }
+__global__ void k_chunk_begin(Ctl *ctl, size_t sb) { at_slot(ctl, sb)->nChunk = 0; }
 
 // ------------------------------------------------------------------------------------------------ set-up kernels
 // Physical initial fields (nslab.js:251-266): kind 0 tgv, 1 tgv2d, 2 abc into phys[0..2] (u, v, w);

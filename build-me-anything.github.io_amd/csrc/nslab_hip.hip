@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
+#include <set>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/nslab_hip.h"
@@ -118,9 +121,15 @@ static bool tiling_for(int N, Tiling &t) {
   return true;
 }
 
+// Raises a kernel's dynamic LDS limit once per (kernel, size); later launches (and stream capture) skip the call.
 template <typename K>
 static void allow_lds(K kernel, int bytes) {
-  if (bytes > 65536) (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (bytes <= 65536) return;
+  static std::mutex mu;
+  static std::set<std::pair<const void *, int>> done;
+  std::lock_guard<std::mutex> lk(mu);
+  if (done.insert({(const void *)kernel, bytes}).second)
+    (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
 // ------------------------------------------------------------------------------------------------ transform set
@@ -134,14 +143,19 @@ struct Xform {
   bool fast = false;
   int fT = 0, fLP = 0;
   double2 *twN = nullptr;   // (cos, sin)(2 pi k / N), k < N
-  double *pmax = nullptr;   // per-workgroup max |omega|^2, |u|^2 of the x pass
 };
 
-static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl);
-static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl);
-static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl);
-static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl);
-static int fast_dispatch_xnl(const Xform &x, double2 *X2, Ctl *ctl);
+// Slot batching of a launch: nz slots (gridDim.z), slot buffers sb bytes apart. {1, 0}: one solve.
+struct Batch {
+  unsigned nz = 1;
+  size_t sb = 0;
+};
+
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt);
 static bool fast_setup(Xform &x);
 
 static Geo geo_truncated(int N) {
@@ -172,8 +186,15 @@ static int zinv_t(const Xform &x, const double2 *U, double2 *X1, int mode, int n
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl) {
-  if (x.fast) return fast_dispatch_zinv(x, U, X1, mode, nout, gc, ctl);
+// The generic engine serves single solves only (pools require a size with compiled kernels).
+static int need_single(const Batch &bt) {
+  return bt.nz == 1 ? NSL_OK : fail(NSL_ERR_STATE, "slot batching needs the size-specialised kernels");
+}
+
+static int zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl,
+                const Batch &bt = Batch()) {
+  if (x.fast) return fast_dispatch_zinv(x, U, X1, mode, nout, gc, ctl, bt);
+  TRY(need_single(bt));
   return x.tl.TNp == 4096 ? zinv_t<4096>(x, U, X1, mode, nout, gc, ctl) : zinv_t<8192>(x, U, X1, mode, nout, gc, ctl);
 }
 
@@ -185,8 +206,9 @@ static int yinv_t(const Xform &x, const double2 *X1, double2 *X2, int ncomp, con
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl) {
-  if (x.fast) return fast_dispatch_yinv(x, X1, X2, ncomp, ctl);
+static int yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt = Batch()) {
+  if (x.fast) return fast_dispatch_yinv(x, X1, X2, ncomp, ctl, bt);
+  TRY(need_single(bt));
   return x.tl.TNp == 4096 ? yinv_t<4096>(x, X1, X2, ncomp, ctl) : yinv_t<8192>(x, X1, X2, ncomp, ctl);
 }
 
@@ -198,8 +220,9 @@ static int yfwd_t(const Xform &x, const double2 *X3, double2 *X4, int ncomp, con
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl) {
-  if (x.fast) return fast_dispatch_yfwd(x, X3, X4, ncomp, ctl);
+static int yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt = Batch()) {
+  if (x.fast) return fast_dispatch_yfwd(x, X3, X4, ncomp, ctl, bt);
+  TRY(need_single(bt));
   return x.tl.TNp == 4096 ? yfwd_t<4096>(x, X3, X4, ncomp, ctl) : yfwd_t<8192>(x, X3, X4, ncomp, ctl);
 }
 
@@ -212,8 +235,9 @@ static int zfwd_t(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, co
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl) {
-  if (x.fast) return fast_dispatch_zfwd(x, X4, OUT, ncomp, ctl);
+static int zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt = Batch()) {
+  if (x.fast) return fast_dispatch_zfwd(x, X4, OUT, ncomp, ctl, bt);
+  TRY(need_single(bt));
   return x.tl.TNp == 4096 ? zfwd_t<4096>(x, X4, OUT, ncomp, ctl) : zfwd_t<8192>(x, X4, OUT, ncomp, ctl);
 }
 
@@ -230,8 +254,9 @@ static int xnl_t(const Xform &x, double2 *X2, Ctl *ctl) {
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int xnl(const Xform &x, double2 *X2, Ctl *ctl) {
-  if (x.fast) return fast_dispatch_xnl(x, X2, ctl);
+static int xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt = Batch()) {
+  if (x.fast) return fast_dispatch_xnl(x, X2, pmax, ctl, bt);
+  TRY(need_single(bt));
   return x.tl.TNx == 4096 ? xnl_t<4096>(x, X2, ctl) : xnl_t<8192>(x, X2, ctl);
 }
 
@@ -273,18 +298,14 @@ static int xform_init(Xform &x, const Geo &g, hipStream_t st) {
     }
     HIP_TRY(hipMalloc(&x.twN, g.N * sizeof(double2)));
     HIP_TRY(hipMemcpy(x.twN, tw.data(), g.N * sizeof(double2), hipMemcpyHostToDevice));
-    // one (max |omega|^2, max |u|^2) slot per x-pass workgroup: at most one workgroup per x row
-    HIP_TRY(hipMalloc(&x.pmax, 2 * (size_t)g.N * g.N * sizeof(double)));
   }
   return NSL_OK;
 }
 static void xform_free(Xform &x) {
   if (x.hp.twd) (void)hipFree(x.hp.twd);
   if (x.twN) (void)hipFree(x.twN);
-  if (x.pmax) (void)hipFree(x.pmax);
   x.hp.twd = nullptr;
   x.twN = nullptr;
-  x.pmax = nullptr;
 }
 
 #include "nsl_fastdisp.hpp"
@@ -292,13 +313,43 @@ static void xform_free(Xform &x) {
 // ------------------------------------------------------------------------------------------------ solver
 #define NSL_RECCAP 64
 
+// Device buffers of one solve, carved from one allocation (a "slot"); a pool places its slots SB bytes apart.
+struct SlotLayout {
+  size_t S, T, ACC, NL, X1, X2, partials, pmax, ctl, recs, bytes;
+};
+
+static SlotLayout slot_layout(const Geo &g, int nblk) {
+  SlotLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t D3 = (size_t)3 * g.D * sizeof(double2);
+  L.S = take(D3); L.T = take(D3); L.ACC = take(D3); L.NL = take(D3);
+  L.X1 = take((size_t)6 * g.N * g.by * g.hp * sizeof(double2));
+  L.X2 = take((size_t)6 * g.N * g.N * g.hp * sizeof(double2));
+  L.partials = take((size_t)nblk * 8 * sizeof(double));
+  L.pmax = take((size_t)2 * g.N * g.N * sizeof(double));   // one (max |omega|^2, max |u|^2) per x-pass workgroup
+  L.ctl = take(sizeof(Ctl));
+  L.recs = take(NSL_RECCAP * sizeof(StepRecDev));
+  L.bytes = (off + 4095) & ~(size_t)4095;
+  return L;
+}
+
+static int nblk_of(const Geo &g) { return (int)((g.D + (long long)NSL_NT * NSL_MPT - 1) / ((long long)NSL_NT * NSL_MPT)); }
+
 struct nsl_solver {
   int N = 0, ic = 0;
   double nu = 0, cfl = 0, dtFixed = 0, icp[4] = {0, 0, 0, 0};
   Xform x;
   hipStream_t st = 0;
+  bool view = false;        // a slot of a pool: stream, transform set and slot memory belong to the pool
+  Batch bt;                 // {1, 0}, or {slots, SB} for the launch object of a pool
+  char *block = nullptr;    // the slot allocation (owned unless view)
   double2 *S = nullptr, *T = nullptr, *ACC = nullptr, *NL = nullptr, *X1 = nullptr, *X2 = nullptr;
-  double *partials = nullptr;
+  double *partials = nullptr, *pmax = nullptr;
   int nblk = 0;
   Ctl *ctl = nullptr;
   StepRecDev *recs = nullptr;
@@ -363,6 +414,14 @@ static int dalloc(nsl_solver *s, T **p, size_t count) {
   return NSL_OK;
 }
 
+static void bind_slot(nsl_solver *s, char *base, const SlotLayout &L) {
+  s->block = base;
+  s->S = (double2 *)(base + L.S); s->T = (double2 *)(base + L.T); s->ACC = (double2 *)(base + L.ACC);
+  s->NL = (double2 *)(base + L.NL); s->X1 = (double2 *)(base + L.X1); s->X2 = (double2 *)(base + L.X2);
+  s->partials = (double *)(base + L.partials); s->pmax = (double *)(base + L.pmax);
+  s->ctl = (Ctl *)(base + L.ctl); s->recs = (StepRecDev *)(base + L.recs);
+}
+
 
 static int prof_collect(nsl_solver *s);
 static int pull_ctl(nsl_solver *s) {
@@ -382,23 +441,24 @@ static void diag_from(const double *d, nsl_rhsdiag &o) {
 // One rhs of U with the fused stage-`stage` update; fmode selects where k_finalize books the sums.
 static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecDev *rec) {
   const Xform &x = s->x;
-  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->ctl)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_ZFWD)); TRY(zfwd(x, s->X1, s->NL, 3, s->ctl)); TRY(prof_end(s));
-  dim3 g((unsigned)s->nblk), b(NSL_NT);
+  const Batch &bt = s->bt;
+  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_ZFWD)); TRY(zfwd(x, s->X1, s->NL, 3, s->ctl, bt)); TRY(prof_end(s));
+  dim3 g((unsigned)s->nblk, 1, bt.nz), b(NSL_NT);
   TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
   switch (stage) {
-    case 1: hipLaunchKernelGGL(k_update<1>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
-    case 2: hipLaunchKernelGGL(k_update<2>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
-    case 3: hipLaunchKernelGGL(k_update<3>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
-    default: hipLaunchKernelGGL(k_update<4>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl); break;
+    case 1: hipLaunchKernelGGL(k_update<1>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl, bt.sb); break;
+    case 2: hipLaunchKernelGGL(k_update<2>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl, bt.sb); break;
+    case 3: hipLaunchKernelGGL(k_update<3>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl, bt.sb); break;
+    default: hipLaunchKernelGGL(k_update<4>, g, b, 0, s->st, s->NL, U, s->S, s->ACC, s->T, x.g, s->partials, s->ctl, bt.sb); break;
   }
   LAUNCH_CHECK();
   TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_FINALIZE));
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSL_NT), 0, s->st, s->partials, s->nblk, fmode, rec, s->ctl);
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bt.nz), dim3(NSL_NT), 0, s->st, s->partials, s->nblk, fmode, rec, s->ctl, bt.sb);
   LAUNCH_CHECK();
   TRY(prof_end(s));
   return NSL_OK;
@@ -407,7 +467,7 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
 // One RK4 step (nslab.js:312-333). Stage 1's rhs is the previous step's last rhs (identical input), already
 // booked in diag[4] with ACC and T prepared; the step ends with the rhs of the new state.
 static int step_enqueue(nsl_solver *s) {
-  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s->st, s->ctl);
+  hipLaunchKernelGGL(k_begin_step, dim3(1, 1, s->bt.nz), dim3(1), 0, s->st, s->ctl, s->bt.sb);
   LAUNCH_CHECK();
   TRY(rhs_enqueue(s, s->T, 2, 1, nullptr));
   TRY(rhs_enqueue(s, s->T, 3, 2, nullptr));
@@ -418,7 +478,7 @@ static int step_enqueue(nsl_solver *s) {
 
 // rhs of the current state outside a step (finishIC, set_state, diagnose): refreshes last, ACC and T.
 static int rhs_current(nsl_solver *s) {
-  hipLaunchKernelGGL(k_set_active, dim3(1), dim3(1), 0, s->st, s->ctl, 1);
+  hipLaunchKernelGGL(k_set_active, dim3(1), dim3(1), 0, s->st, s->ctl, 1, (size_t)0);
   LAUNCH_CHECK();
   TRY(rhs_enqueue(s, s->S, 1, 4, nullptr));
   TRY(pull_ctl(s));
@@ -556,16 +616,28 @@ static int set_ic(nsl_solver *s, int ic, const double *icp) {
 
 static void solver_free(nsl_solver *s) {
   if (!s) return;
-  (void)hipFree(s->S); (void)hipFree(s->T); (void)hipFree(s->ACC); (void)hipFree(s->NL); (void)hipFree(s->X1); (void)hipFree(s->X2);
-  (void)hipFree(s->partials); (void)hipFree(s->ctl); (void)hipFree(s->recs);
   if (s->phys) (void)hipFree(s->phys);
   if (s->dpart) (void)hipFree(s->dpart);
   if (s->hrecs) (void)hipHostFree(s->hrecs);
   if (s->hctl) (void)hipHostFree(s->hctl);
-  xform_free(s->x);
   for (auto e : s->evs) (void)hipEventDestroy(e);
-  if (s->st) (void)hipStreamDestroy(s->st);
+  if (!s->view) {
+    if (s->block) (void)hipFree(s->block);
+    xform_free(s->x);
+    if (s->st) (void)hipStreamDestroy(s->st);
+  }
   delete s;
+}
+
+// Host-side state of a solve whose device buffers are bound: pinned mirrors and the initial control block.
+static int solver_host_init(nsl_solver *s) {
+  if (hipHostMalloc((void **)&s->hrecs, NSL_RECCAP * sizeof(StepRecDev)) != hipSuccess ||
+      hipHostMalloc((void **)&s->hctl, sizeof(Ctl)) != hipSuccess)
+    return fail(NSL_ERR_HIP, "hipHostMalloc failed");
+  memset(s->hctl, 0, sizeof(Ctl));
+  Ctl *c = s->hctl;
+  c->N = s->N; c->kc = s->N / 3; c->nu = s->nu; c->cfl = s->cfl; c->dtFixed = s->dtFixed; c->E0ref = 1.0;
+  return push_ctl(s);
 }
 
 static int create_impl(const nsl_opts *o, nsl_solver **out) {
@@ -582,29 +654,16 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
   if (e != hipSuccess) { delete s; return fail(NSL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
   int r = xform_init(s->x, geo_truncated(N), s->st);
   if (r != NSL_OK) { solver_free(s); return r; }
-  const Geo &g = s->x.g;
-  const size_t D3 = (size_t)3 * g.D;
-  const size_t X1n = (size_t)6 * N * g.by * g.hp, X2n = (size_t)6 * N * N * g.hp;
-  s->nblk = (int)((g.D + (long long)NSL_NT * NSL_MPT - 1) / ((long long)NSL_NT * NSL_MPT));
-  if ((r = dalloc(s, &s->S, D3)) || (r = dalloc(s, &s->T, D3)) || (r = dalloc(s, &s->ACC, D3)) ||
-      (r = dalloc(s, &s->NL, D3)) || (r = dalloc(s, &s->X1, X1n)) || (r = dalloc(s, &s->X2, X2n)) ||
-      (r = dalloc(s, &s->partials, (size_t)s->nblk * 8)) || (r = dalloc(s, &s->ctl, 1)) ||
-      (r = dalloc(s, &s->recs, NSL_RECCAP))) {
-    solver_free(s);
-    return r;
-  }
-  // the padding slots of every row stay zero for the life of the solver
-  for (double2 *p : {s->S, s->T, s->ACC, s->NL})
-    if (hipMemset(p, 0, D3 * sizeof(double2)) != hipSuccess) { solver_free(s); return fail(NSL_ERR_HIP, "hipMemset failed"); }
-  if (hipHostMalloc((void **)&s->hrecs, NSL_RECCAP * sizeof(StepRecDev)) != hipSuccess ||
-      hipHostMalloc((void **)&s->hctl, sizeof(Ctl)) != hipSuccess) {
-    solver_free(s);
-    return fail(NSL_ERR_HIP, "hipHostMalloc failed");
-  }
-  memset(s->hctl, 0, sizeof(Ctl));
-  Ctl *c = s->hctl;
-  c->N = N; c->kc = N / 3; c->nu = s->nu; c->cfl = s->cfl; c->dtFixed = s->dtFixed; c->E0ref = 1.0;
-  if ((r = push_ctl(s)) != NSL_OK) { solver_free(s); return r; }
+  s->nblk = nblk_of(s->x.g);
+  const SlotLayout L = slot_layout(s->x.g, s->nblk);
+  char *base = nullptr;
+  if ((r = dalloc(s, &base, L.bytes)) != NSL_OK) { solver_free(s); return r; }
+  bind_slot(s, base, L);
+  // the padding slots of every row stay zero for the life of the solver. On the solver's stream: a non-blocking
+  // stream does not wait for work on the null stream, so a plain hipMemset could still be clearing the block
+  // (control block included) while the set-up kernels run.
+  if (hipMemsetAsync(base, 0, L.bytes, s->st) != hipSuccess) { solver_free(s); return fail(NSL_ERR_HIP, "hipMemset failed"); }
+  if ((r = solver_host_init(s)) != NSL_OK) { solver_free(s); return r; }
   if ((r = set_ic(s, o->ic, o->icp)) != NSL_OK) { solver_free(s); return r; }
   *out = s;
   return NSL_OK;
@@ -621,7 +680,7 @@ static int run_impl(nsl_solver *s, int64_t n, int has_tend, double tend, nsl_ste
   int64_t enq = 0, done = 0;
   while (enq < n) {
     const int chunk = (int)std::min<int64_t>(NSL_RECCAP, n - enq);
-    hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(1), 0, s->st, s->ctl);
+    hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(1), 0, s->st, s->ctl, (size_t)0);
     LAUNCH_CHECK();
     for (int i = 0; i < chunk; i++) TRY(step_enqueue(s));
     HIP_TRY(hipMemcpyAsync(s->hrecs, s->recs, chunk * sizeof(StepRecDev), hipMemcpyDeviceToHost, s->st));
@@ -644,6 +703,240 @@ static int run_impl(nsl_solver *s, int64_t n, int has_tend, double tend, nsl_ste
   diag_from(c->diag[4], s->last);
   if (done > 0) s->diag_valid = 0;
   if (n_done) *n_done = done;
+  return NSL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------ pools
+// A pool holds `slots` solves of one N in one allocation, slot i at base + i*SB. One launch of every pass covers
+// all slots (gridDim.z = slots); each slot's control block decides whether its workgroups work or return, so
+// solves join and leave between chunks without changing the launches, and a slot's arithmetic (launch shape per
+// slot, reduction order) is the same as a single solve's whatever the other slots hold. One RK4 step of the pool
+// is captured once as a hipGraph and replayed.
+struct PoolStage {
+  Ctl ctl;
+  StepRecDev recs[NSL_RECCAP];
+};
+
+// Copies each slot's control block and step records into a contiguous staging array (one copy to the host).
+__global__ void k_pool_gather(const char *__restrict__ base, size_t sb, size_t offCtl, size_t offRecs, int nrec,
+                              PoolStage *__restrict__ out) {
+  const char *slot = base + (size_t)blockIdx.x * sb;
+  const double *c = (const double *)(slot + offCtl);
+  double *oc = (double *)&out[blockIdx.x].ctl;
+  for (int i = threadIdx.x; i < (int)(sizeof(Ctl) / sizeof(double)); i += blockDim.x) oc[i] = c[i];
+  const double *r = (const double *)(slot + offRecs);
+  double *orr = (double *)out[blockIdx.x].recs;
+  const int nr = nrec * (int)(sizeof(StepRecDev) / sizeof(double));
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) orr[i] = r[i];
+}
+
+struct nsl_pool {
+  int N = 0, C = 0, nblk = 0;
+  Xform x;
+  hipStream_t st = 0;
+  SlotLayout L;
+  char *base = nullptr;
+  nsl_solver *lead = nullptr;            // the launch object: slot 0's buffers, Batch {C, SB}
+  std::vector<nsl_solver *> slots;       // one view per slot
+  std::vector<int> used;
+  PoolStage *dstage = nullptr, *hstage = nullptr;
+  int lastChunk = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graphed = 0;                       // 1: steps replay the captured graph; 0: eager launches
+  int64_t bytes = 0;
+};
+
+static int64_t slot_bytes_of(int N) {
+  const Geo g = geo_truncated(N);
+  return (int64_t)slot_layout(g, nblk_of(g)).bytes;
+}
+
+static void pool_free(nsl_pool *p) {
+  if (!p) return;
+  if (p->st) (void)hipStreamSynchronize(p->st);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (nsl_solver *v : p->slots) solver_free(v);
+  solver_free(p->lead);
+  if (p->dstage) (void)hipFree(p->dstage);
+  if (p->hstage) (void)hipHostFree(p->hstage);
+  if (p->base) (void)hipFree(p->base);
+  xform_free(p->x);
+  if (p->st) (void)hipStreamDestroy(p->st);
+  delete p;
+}
+
+static nsl_solver *pool_view(nsl_pool *p, char *base, Batch bt) {
+  nsl_solver *v = new nsl_solver();
+  v->view = true;
+  v->N = p->N;
+  v->x = p->x;
+  v->st = p->st;
+  v->bt = bt;
+  v->nblk = p->nblk;
+  bind_slot(v, base, p->L);
+  return v;
+}
+
+static int pool_create_impl(int N, int C, nsl_pool **out) {
+  if (!out) return fail(NSL_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::vector<int> f;
+  if (factorize_host(N, f) < 0) return fail(NSL_ERR_ARG, bad_n_msg(N));
+  if (C < 1 || C > 65535) return fail(NSL_ERR_ARG, "slots must be in 1..65535");
+  nsl_pool *p = new nsl_pool();
+  p->N = N; p->C = C;
+  hipError_t e = hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete p; return fail(NSL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  int r = xform_init(p->x, geo_truncated(N), p->st);
+  if (r == NSL_OK && !p->x.fast) r = fail(NSL_ERR_ARG, "pools need a grid size with compiled kernels (16..1024, 2^a or 2^a*3)");
+  if (r != NSL_OK) { pool_free(p); return r; }
+  p->nblk = nblk_of(p->x.g);
+  p->L = slot_layout(p->x.g, p->nblk);
+  const size_t total = (size_t)C * p->L.bytes;
+  if (hipMalloc((void **)&p->base, total) != hipSuccess) {
+    (void)hipGetLastError();
+    pool_free(p);
+    return fail(NSL_ERR_HIP, "hipMalloc of " + std::to_string(total) + " bytes for " + std::to_string(C) + " slots failed");
+  }
+  p->bytes = (int64_t)total;
+  // zero: padding stays zero for the pool's life, and every control block starts idle (nTarget = 0)
+  if (hipMemsetAsync(p->base, 0, total, p->st) != hipSuccess) { pool_free(p); return fail(NSL_ERR_HIP, "hipMemset failed"); }
+  if (hipMalloc((void **)&p->dstage, (size_t)C * sizeof(PoolStage)) != hipSuccess ||
+      hipHostMalloc((void **)&p->hstage, (size_t)C * sizeof(PoolStage)) != hipSuccess) {
+    pool_free(p);
+    return fail(NSL_ERR_HIP, "staging allocation failed");
+  }
+  p->lead = pool_view(p, p->base, Batch{(unsigned)C, p->L.bytes});
+  for (int i = 0; i < C; i++) p->slots.push_back(pool_view(p, p->base + (size_t)i * p->L.bytes, Batch()));
+  p->used.assign(C, 0);
+  // one idle step (every slot inactive: each kernel returns at once) sets the launch attributes, then the
+  // step is captured
+  if ((r = step_enqueue(p->lead)) != NSL_OK) { pool_free(p); return r; }
+  HIP_TRY(hipStreamSynchronize(p->st));
+  if (hipStreamBeginCapture(p->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    const int rc = step_enqueue(p->lead);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(p->st, &g);
+    if (rc == NSL_OK && ec == hipSuccess && g != nullptr &&
+        hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0) == hipSuccess) {
+      p->graph = g;
+      p->graphed = 1;
+    } else if (g != nullptr) {
+      (void)hipGraphDestroy(g);
+    }
+  }
+  (void)hipGetLastError();
+  *out = p;
+  return NSL_OK;
+}
+
+static int pool_slot_check(nsl_pool *p, int slot, bool must_be_used) {
+  if (!p) return fail(NSL_ERR_ARG, "null pool");
+  if (slot < 0 || slot >= p->C) return fail(NSL_ERR_ARG, "slot out of range");
+  if (must_be_used && !p->used[slot]) return fail(NSL_ERR_STATE, "slot is free");
+  return NSL_OK;
+}
+
+static int pool_admit_impl(nsl_pool *p, const nsl_opts *o, int *slot) {
+  if (!p || !o || !slot) return fail(NSL_ERR_ARG, "null argument");
+  if (o->N != p->N) return fail(NSL_ERR_ARG, "grid size " + std::to_string(o->N) + " does not match the pool's " + std::to_string(p->N));
+  if (o->ic < 0 || o->ic > 4) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
+  int i = 0;
+  while (i < p->C && p->used[i]) i++;
+  if (i == p->C) return fail(NSL_ERR_STATE, "pool is full");
+  nsl_solver *v = p->slots[i];
+  v->nu = o->nu; v->cfl = o->cfl; v->dtFixed = o->dt;
+  v->t = 0; v->step = 0; v->dt = 0; v->divMax = 0; v->diag_valid = 0;
+  if (!v->hctl) {
+    TRY(solver_host_init(v));
+  } else {
+    memset(v->hctl, 0, sizeof(Ctl));
+    Ctl *c = v->hctl;
+    c->N = v->N; c->kc = v->N / 3; c->nu = v->nu; c->cfl = v->cfl; c->dtFixed = v->dtFixed; c->E0ref = 1.0;
+    TRY(push_ctl(v));
+  }
+  TRY(set_ic(v, o->ic, o->icp));
+  p->used[i] = 1;
+  *slot = i;
+  return NSL_OK;
+}
+
+static int pool_release_impl(nsl_pool *p, int slot) {
+  TRY(pool_slot_check(p, slot, true));
+  nsl_solver *v = p->slots[slot];
+  TRY(pull_ctl(v));
+  v->hctl->nTarget = v->hctl->nDone;   // idle from the next step on
+  TRY(push_ctl(v));
+  if (v->phys) { (void)hipFree(v->phys); v->phys = nullptr; }
+  if (v->dpart) { (void)hipFree(v->dpart); v->dpart = nullptr; }
+  p->used[slot] = 0;
+  return NSL_OK;
+}
+
+// The next chunk(s) of this slot run up to n steps (and stop at tEnd when has_tend), as run() would.
+static int pool_set_target_impl(nsl_pool *p, int slot, int64_t n, int has_tend, double tend) {
+  TRY(pool_slot_check(p, slot, true));
+  nsl_solver *v = p->slots[slot];
+  TRY(pull_ctl(v));
+  Ctl *c = v->hctl;
+  c->nDone = 0; c->nTarget = n > 0 ? n : 0; c->hasTEnd = has_tend ? 1 : 0; c->tEnd = tend; c->nChunk = 0;
+  c->t = v->t; c->step = v->step; c->E0ref = v->E0;
+  return push_ctl(v);
+}
+
+// Enqueues up to `steps` (<= NSL_RECCAP) steps of every slot on the pool's stream; returns without waiting.
+static int pool_launch_impl(nsl_pool *p, int steps) {
+  if (!p) return fail(NSL_ERR_ARG, "null pool");
+  if (steps < 1 || steps > NSL_RECCAP) return fail(NSL_ERR_ARG, "steps per launch must be in 1..64");
+  hipLaunchKernelGGL(k_chunk_begin, dim3(1, 1, p->C), dim3(1), 0, p->st, p->lead->ctl, p->L.bytes);
+  LAUNCH_CHECK();
+  for (int i = 0; i < steps; i++) {
+    if (p->graphed) HIP_TRY(hipGraphLaunch(p->gexec, p->st));
+    else TRY(step_enqueue(p->lead));
+  }
+  hipLaunchKernelGGL(k_pool_gather, dim3(p->C), dim3(256), 0, p->st, (const char *)p->base, p->L.bytes, p->L.ctl,
+                     p->L.recs, steps, p->dstage);
+  LAUNCH_CHECK();
+  HIP_TRY(hipMemcpyAsync(p->hstage, p->dstage, (size_t)p->C * sizeof(PoolStage), hipMemcpyDeviceToHost, p->st));
+  p->lastChunk = steps;
+  return NSL_OK;
+}
+
+// Waits for the launched chunk and books it: per slot, the steps done (n_done[slot]) and their records
+// (recs[slot * recs_per_slot + i], if recs); *n_running counts slots with steps left.
+static int pool_collect_impl(nsl_pool *p, nsl_steprec *recs, int recs_per_slot, int64_t *n_done, int *n_running) {
+  if (!p) return fail(NSL_ERR_ARG, "null pool");
+  HIP_TRY(hipStreamSynchronize(p->st));
+  int running = 0;
+  for (int i = 0; i < p->C; i++) {
+    if (n_done) n_done[i] = 0;
+    if (!p->used[i]) continue;
+    nsl_solver *v = p->slots[i];
+    const PoolStage &P = p->hstage[i];
+    memcpy(v->hctl, &P.ctl, sizeof(Ctl));
+    const Ctl &c = P.ctl;
+    const int got = p->lastChunk > 0 ? (int)c.nChunk : 0;
+    for (int k = 0; k < got; k++) {
+      if (recs && k < recs_per_slot) {
+        const StepRecDev &R = P.recs[k];
+        nsl_steprec &o = recs[(size_t)i * recs_per_slot + k];
+        o.t = R.t; o.dt = R.dt; o.step = R.step; o.ebal = R.ebal; o.zbal = R.zbal;
+        diag_from(R.last, o.last);
+        diag_from(R.d0, o.d0);
+      }
+    }
+    if (got > 0) {
+      v->t = c.t; v->step = c.step; v->dt = c.dt;
+      diag_from(c.diag[4], v->last);
+      v->diag_valid = 0;
+    }
+    if (n_done) n_done[i] = got;
+    if (c.nDone < c.nTarget && !(c.hasTEnd && c.t >= c.tEnd - 1e-12)) running++;
+  }
+  p->lastChunk = 0;
+  if (n_running) *n_running = running;
   return NSL_OK;
 }
 
@@ -776,7 +1069,10 @@ int nsl_set_ic(nsl_solver *s, int32_t ic, const double icp[4]) {
   return set_ic(s, ic, icp);
 }
 
-void nsl_destroy(nsl_solver *s) { solver_free(s); }
+void nsl_destroy(nsl_solver *s) {
+  if (s && s->view) return;   // a pool's slot: released with nsl_pool_release, freed with the pool
+  solver_free(s);
+}
 
 int nsl_run(nsl_solver *s, int64_t n, int32_t has_tend, double tend, nsl_steprec *recs, int64_t *n_done) {
   if (!s) return fail(NSL_ERR_ARG, "null solver");
@@ -897,6 +1193,57 @@ int nsl_field(nsl_solver *s, int32_t which, double *out) {
 int nsl_exact_error(nsl_solver *s, double out[2]) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
   return exact_error_impl(s, out);
+}
+
+int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out) { return pool_create_impl(N, slots, out); }
+
+void nsl_pool_destroy(nsl_pool *p) { pool_free(p); }
+
+int64_t nsl_pool_slot_bytes(int32_t N) {
+  std::vector<int> f;
+  if (factorize_host(N, f) < 0) return fail(NSL_ERR_ARG, bad_n_msg(N));
+  return slot_bytes_of(N);
+}
+
+int nsl_pool_info(nsl_pool *p, int32_t *N, int32_t *slots, int32_t *used, int32_t *graphed, int64_t *bytes) {
+  if (!p) return fail(NSL_ERR_ARG, "null pool");
+  int u = 0;
+  for (int v : p->used) u += v;
+  if (N) *N = p->N;
+  if (slots) *slots = p->C;
+  if (used) *used = u;
+  if (graphed) *graphed = p->graphed;
+  if (bytes) *bytes = p->bytes;
+  return NSL_OK;
+}
+
+int nsl_pool_admit(nsl_pool *p, const nsl_opts *opts, int32_t *slot) {
+  int s = -1;
+  const int r = pool_admit_impl(p, opts, &s);
+  if (slot) *slot = s;
+  return r;
+}
+
+int nsl_pool_release(nsl_pool *p, int32_t slot) { return pool_release_impl(p, slot); }
+
+int nsl_pool_slot(nsl_pool *p, int32_t slot, nsl_solver **view) {
+  TRY(pool_slot_check(p, slot, true));
+  if (!view) return fail(NSL_ERR_ARG, "null argument");
+  *view = p->slots[slot];
+  return NSL_OK;
+}
+
+int nsl_pool_set_target(nsl_pool *p, int32_t slot, int64_t n, int32_t has_tend, double tend) {
+  return pool_set_target_impl(p, slot, n, has_tend, tend);
+}
+
+int nsl_pool_launch(nsl_pool *p, int32_t steps) { return pool_launch_impl(p, steps); }
+
+int nsl_pool_collect(nsl_pool *p, nsl_steprec *recs, int32_t recs_per_slot, int64_t *n_done, int32_t *n_running) {
+  int nr = 0;
+  const int r = pool_collect_impl(p, recs, recs_per_slot, n_done, &nr);
+  if (n_running) *n_running = nr;
+  return r;
 }
 
 }  // extern "C"

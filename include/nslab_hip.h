@@ -130,6 +130,33 @@ int64_t nsl_device_bytes(const nsl_solver *s);
 int nsl_profile(nsl_solver *s, int32_t on);
 int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count);
 
+/* Slot pools: many solves of one N resident on the device and stepped together. The reference runs the levels of
+ * a study one createSolver/step loop at a time (nslab-worker.js:42-49, startLevel/loop) and the GPU runner one
+ * solve per process; a pool replaces those loops when several solves are in flight. Every pass is one launch over
+ * all slots (gridDim.z = slots); a slot's arithmetic is the single solve's, so its results do not depend on what
+ * the other slots hold. One RK4 step of the pool is a captured hipGraph.
+ *   nsl_pool_slot_bytes(N)         device bytes one slot of grid N occupies (admission by HBM footprint)
+ *   nsl_pool_create(N, slots, &p)  allocates `slots` slots, all free
+ *   nsl_pool_admit(p, opts, &i)    createSolver(opts) into a free slot (opts.N must equal the pool's N)
+ *   nsl_pool_slot(p, i, &s)        the slot as a solver handle: every nsl_* solver call above works on it
+ *                                  (state, clock, last, diagnose, spectrum, nsl_run); nsl_destroy ignores it
+ *   nsl_pool_set_target(p, i, n, has_tend, tend)   the slot runs n more steps (stopping at tend), as run(n, tEnd)
+ *   nsl_pool_launch(p, k)          enqueues k <= 64 steps of every slot with steps left; returns at once
+ *   nsl_pool_collect(p, recs, cap, n_done, &running)   waits; per slot i: n_done[i] steps and their records at
+ *                                  recs[i*cap ..] (recs may be null); running = slots with steps left
+ *   nsl_pool_release(p, i)         frees the slot */
+typedef struct nsl_pool nsl_pool;
+int64_t nsl_pool_slot_bytes(int32_t N);
+int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out);
+void nsl_pool_destroy(nsl_pool *p);
+int nsl_pool_info(nsl_pool *p, int32_t *N, int32_t *slots, int32_t *used, int32_t *graphed, int64_t *bytes);
+int nsl_pool_admit(nsl_pool *p, const nsl_opts *opts, int32_t *slot);
+int nsl_pool_slot(nsl_pool *p, int32_t slot, nsl_solver **view);
+int nsl_pool_set_target(nsl_pool *p, int32_t slot, int64_t n, int32_t has_tend, double tend);
+int nsl_pool_launch(nsl_pool *p, int32_t steps);
+int nsl_pool_collect(nsl_pool *p, nsl_steprec *recs, int32_t recs_per_slot, int64_t *n_done, int32_t *n_running);
+int nsl_pool_release(nsl_pool *p, int32_t slot);
+
 /* NS.fft1 / NS.Spectral (nslab.js:46-155) on the GPU: `lines` independent length-N lines (split re/im, line-major)
  * transformed in place, sign -1 forward, +1 inverse, unnormalised. */
 int nsl_fft1(int32_t N, int32_t sign, int64_t lines, double *re, double *im);

@@ -71,6 +71,21 @@ def lib():
         L.nsl_spectral_forward.argtypes = [ctypes.c_int32, dp, dp, dp]
         L.nsl_spectral_inverse.argtypes = [ctypes.c_int32, dp, dp, dp]
         L.nsl_factorize.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.nsl_pool_slot_bytes.argtypes = [ctypes.c_int32]
+        L.nsl_pool_slot_bytes.restype = ctypes.c_int64
+        L.nsl_pool_create.argtypes = [ctypes.c_int32, ctypes.c_int32, pp]
+        L.nsl_pool_destroy.argtypes = [vp]
+        L.nsl_pool_destroy.restype = None
+        L.nsl_pool_info.argtypes = [vp, i32p, i32p, i32p, i32p, i64p]
+        L.nsl_pool_admit.argtypes = [vp, ctypes.POINTER(Opts), i32p]
+        L.nsl_pool_slot.argtypes = [vp, ctypes.c_int32, pp]
+        L.nsl_pool_set_target.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_double]
+        L.nsl_pool_launch.argtypes = [vp, ctypes.c_int32]
+        L.nsl_pool_collect.argtypes = [vp, ctypes.POINTER(StepRec), ctypes.c_int32, i64p, i32p]
+        L.nsl_pool_release.argtypes = [vp, ctypes.c_int32]
         L.nsl_last_error.restype = ctypes.c_char_p
         L.nsl_version.restype = ctypes.c_char_p
         _lib = L
@@ -89,6 +104,18 @@ def check(rc):
 
 def _p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_opts(N, Re=None, nu=None, cfl=None, dt=None, ic="tgv", icParams=None):
+    o = Opts()
+    o.N = int(N)
+    o.ic = IC[ic] if ic in IC else 99
+    o.nu = float(nu) if nu is not None else 1.0 / float(Re or 100)
+    o.cfl = float(cfl) if cfl else 0.4
+    o.dt = float(dt) if dt else 0.0
+    for i, v in enumerate(ic_params(ic, icParams) if ic in IC else [0, 0, 0, 0]):
+        o.icp[i] = v
+    return o
 
 
 def ic_params(kind, prm=None):
@@ -112,17 +139,17 @@ def ic_params(kind, prm=None):
 class Solver:
     """createSolver(opts) through the C ABI, with the reference's st/series bookkeeping (nslab.js:188-190, 300-302)."""
 
-    def __init__(self, N=32, Re=None, nu=None, cfl=None, dt=None, ic="tgv", icParams=None):
-        o = Opts()
-        o.N = int(N)
-        o.ic = IC[ic] if ic in IC else 99
-        o.nu = float(nu) if nu is not None else 1.0 / float(Re or 100)
-        o.cfl = float(cfl) if cfl else 0.4
-        o.dt = float(dt) if dt else 0.0
-        for i, v in enumerate(ic_params(ic, icParams) if ic in IC else [0, 0, 0, 0]):
-            o.icp[i] = v
+    def __init__(self, N=32, Re=None, nu=None, cfl=None, dt=None, ic="tgv", icParams=None, pool=None):
+        o = make_opts(N, Re, nu, cfl, dt, ic, icParams)
         h = ctypes.c_void_p()
-        check(lib().nsl_create(ctypes.byref(o), ctypes.byref(h)))
+        self.pool, self.slot = pool, None
+        if pool is None:
+            check(lib().nsl_create(ctypes.byref(o), ctypes.byref(h)))
+        else:   # a slot of a pool: createSolver into the slot, then the slot's solver handle
+            sl = ctypes.c_int32(-1)
+            check(lib().nsl_pool_admit(pool.h, ctypes.byref(o), ctypes.byref(sl)))
+            self.slot = sl.value
+            check(lib().nsl_pool_slot(pool.h, self.slot, ctypes.byref(h)))
         self.h = h
         self.N = o.N
         self.nu = o.nu
@@ -141,7 +168,10 @@ class Solver:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().nsl_destroy(self.h)
+            if self.pool is None:
+                lib().nsl_destroy(self.h)
+            elif self.pool.h:
+                check(lib().nsl_pool_release(self.pool.h, self.slot))
             self.h = None
 
     def __del__(self):
@@ -149,6 +179,15 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def _book(self, recs, got):
+        for i in range(got):
+            r = recs[i]
+            d0 = r.d0.asdict()
+            self.maxTnl = max(self.maxTnl, abs(d0["Tnl"]) / max(d0["eps"], 1e-300))
+            self._record(r.t, r.dt, r.last.asdict())
+            self.series["ebal"].append(r.ebal); self.series["zbal"].append(r.zbal)
+            self.maxEbal = max(self.maxEbal, r.ebal); self.maxZbal = max(self.maxZbal, r.zbal)
 
     def run(self, n, tEnd=None, chunk=1 << 20):
         total = 0
@@ -158,13 +197,7 @@ class Solver:
             got = ctypes.c_int64(0)
             check(lib().nsl_run(self.h, k, 0 if tEnd is None else 1, 0.0 if tEnd is None else float(tEnd), recs,
                                 ctypes.byref(got)))
-            for i in range(got.value):
-                r = recs[i]
-                d0 = r.d0.asdict()
-                self.maxTnl = max(self.maxTnl, abs(d0["Tnl"]) / max(d0["eps"], 1e-300))
-                self._record(r.t, r.dt, r.last.asdict())
-                self.series["ebal"].append(r.ebal); self.series["zbal"].append(r.zbal)
-                self.maxEbal = max(self.maxEbal, r.ebal); self.maxZbal = max(self.maxZbal, r.zbal)
+            self._book(recs, got.value)
             total += got.value
             if got.value < k:
                 break
@@ -271,3 +304,77 @@ def factorize(N):
     f = (ctypes.c_int32 * 40)()
     k = check(lib().nsl_factorize(int(N), f, 40))
     return list(f[:k])
+
+
+class Pool:
+    """A slot pool (nsl_pool_*): solves of one N stepped together, one launch per pass over all slots."""
+
+    CHUNK = 64
+
+    def __init__(self, N, slots):
+        h = ctypes.c_void_p()
+        check(lib().nsl_pool_create(int(N), int(slots), ctypes.byref(h)))
+        self.h, self.N, self.C = h, int(N), int(slots)
+        self.members = {}
+
+    @staticmethod
+    def slot_bytes(N):
+        return check(lib().nsl_pool_slot_bytes(int(N)))
+
+    def info(self):
+        v = [ctypes.c_int32() for _ in range(4)] + [ctypes.c_int64()]
+        check(lib().nsl_pool_info(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(N=v[0].value, slots=v[1].value, used=v[2].value, graphed=v[3].value, bytes=v[4].value)
+
+    def admit(self, **kw):
+        s = Solver(N=self.N, pool=self, **kw)
+        self.members[s.slot] = s
+        return s
+
+    def release(self, s):
+        self.members.pop(s.slot, None)
+        s.close()
+
+    def set_target(self, s, n, tEnd=None):
+        check(lib().nsl_pool_set_target(self.h, s.slot, int(n), 0 if tEnd is None else 1,
+                                        0.0 if tEnd is None else float(tEnd)))
+
+    def launch(self, k=None):
+        check(lib().nsl_pool_launch(self.h, int(k or self.CHUNK)))
+
+    def collect(self, k=None):
+        """Waits for the launched chunk; books each member's records; returns (steps done per slot, running)."""
+        k = int(k or self.CHUNK)
+        recs = (StepRec * (self.C * k))()
+        done = (ctypes.c_int64 * self.C)()
+        running = ctypes.c_int32(0)
+        check(lib().nsl_pool_collect(self.h, recs, k, done, ctypes.byref(running)))
+        for slot, s in self.members.items():
+            if done[slot]:
+                base = ctypes.addressof(recs) + slot * k * ctypes.sizeof(StepRec)
+                s._book((StepRec * k).from_address(base), done[slot])
+        return list(done), running.value
+
+    def run_all(self, k=None):
+        """Launch/collect until no slot has steps left; returns the number of chunks."""
+        n = 0
+        while True:
+            self.launch(k)
+            _, running = self.collect(k)
+            n += 1
+            if running == 0:
+                return n
+
+    def close(self):
+        if getattr(self, "h", None):
+            for s in list(self.members.values()):
+                s.h = None
+            self.members = {}
+            lib().nsl_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -36,3 +36,18 @@ def test_host_side_errors_without_gpu():
         raise AssertionError("expected an error")
     except capi.NslError as e:
         assert "2^a·3^b ≥ 4 (got 7)" in str(e)
+
+
+def test_pool_slot_bytes_cover_the_solver_buffers():
+    """One slot holds S, T, ACC, NL (3 D each), X1 (6 N b hp), X2 (6 N^2 hp) complex FP64 plus small control data."""
+    prev = 0
+    for N in (32, 48, 64, 96, 128, 192, 256, 512):
+        kc = N // 3
+        hx, b = kc + 1, 2 * kc + 1
+        hp = (hx + 7) & ~7
+        D = b * b * hp
+        big = 16 * (12 * D + 6 * N * b * hp + 6 * N * N * hp)
+        got = capi.Pool.slot_bytes(N)
+        assert big <= got <= big * 1.05 + (1 << 20), N
+        assert got % 4096 == 0 and got > prev
+        prev = got
