@@ -739,6 +739,9 @@ struct nsl_pool {
   nsl_solver *lead = nullptr;            // the launch object: slot 0's buffers, Batch {C, SB}
   std::vector<nsl_solver *> slots;       // one view per slot
   std::vector<int> used;
+  // per-slot generation, bumped by admit / set_target / release; launch_gen is its value at the last launch, so
+  // collect can tell slots changed while a chunk was in flight (their staged control block predates the change)
+  std::vector<uint64_t> gen, launch_gen;
   PoolStage *dstage = nullptr, *hstage = nullptr;
   int lastChunk = 0;
   hipGraph_t graph = nullptr;
@@ -811,6 +814,8 @@ static int pool_create_impl(int N, int C, nsl_pool **out) {
   p->lead = pool_view(p, p->base, Batch{(unsigned)C, p->L.bytes});
   for (int i = 0; i < C; i++) p->slots.push_back(pool_view(p, p->base + (size_t)i * p->L.bytes, Batch()));
   p->used.assign(C, 0);
+  p->gen.assign(C, 0);
+  p->launch_gen.assign(C, 0);
   // one idle step (every slot inactive: each kernel returns at once) sets the launch attributes, then the
   // step is captured
   if ((r = step_enqueue(p->lead)) != NSL_OK) { pool_free(p); return r; }
@@ -859,6 +864,7 @@ static int pool_admit_impl(nsl_pool *p, const nsl_opts *o, int *slot) {
   }
   TRY(set_ic(v, o->ic, o->icp));
   p->used[i] = 1;
+  p->gen[i]++;
   *slot = i;
   return NSL_OK;
 }
@@ -872,6 +878,7 @@ static int pool_release_impl(nsl_pool *p, int slot) {
   if (v->phys) { (void)hipFree(v->phys); v->phys = nullptr; }
   if (v->dpart) { (void)hipFree(v->dpart); v->dpart = nullptr; }
   p->used[slot] = 0;
+  p->gen[slot]++;
   return NSL_OK;
 }
 
@@ -883,6 +890,7 @@ static int pool_set_target_impl(nsl_pool *p, int slot, int64_t n, int has_tend, 
   Ctl *c = v->hctl;
   c->nDone = 0; c->nTarget = n > 0 ? n : 0; c->hasTEnd = has_tend ? 1 : 0; c->tEnd = tend; c->nChunk = 0;
   c->t = v->t; c->step = v->step; c->E0ref = v->E0;
+  p->gen[slot]++;
   return push_ctl(v);
 }
 
@@ -901,6 +909,7 @@ static int pool_launch_impl(nsl_pool *p, int steps) {
   LAUNCH_CHECK();
   HIP_TRY(hipMemcpyAsync(p->hstage, p->dstage, (size_t)p->C * sizeof(PoolStage), hipMemcpyDeviceToHost, p->st));
   p->lastChunk = steps;
+  p->launch_gen = p->gen;
   return NSL_OK;
 }
 
@@ -914,6 +923,11 @@ static int pool_collect_impl(nsl_pool *p, nsl_steprec *recs, int recs_per_slot, 
     if (n_done) n_done[i] = 0;
     if (!p->used[i]) continue;
     nsl_solver *v = p->slots[i];
+    if (p->gen[i] != p->launch_gen[i]) {   // admitted or re-targeted after the launch: nothing of it was staged
+      const Ctl &c = *v->hctl;
+      if (c.nDone < c.nTarget && !(c.hasTEnd && c.t >= c.tEnd - 1e-12)) running++;
+      continue;
+    }
     const PoolStage &P = p->hstage[i];
     memcpy(v->hctl, &P.ctl, sizeof(Ctl));
     const Ctl &c = P.ctl;
@@ -1195,6 +1209,14 @@ int nsl_exact_error(nsl_solver *s, double out[2]) {
   return exact_error_impl(s, out);
 }
 
+int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (int64_t)f;
+  if (total_bytes) *total_bytes = (int64_t)t;
+  return NSL_OK;
+}
+
 int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out) { return pool_create_impl(N, slots, out); }
 
 void nsl_pool_destroy(nsl_pool *p) { pool_free(p); }
@@ -1238,6 +1260,14 @@ int nsl_pool_set_target(nsl_pool *p, int32_t slot, int64_t n, int32_t has_tend, 
 }
 
 int nsl_pool_launch(nsl_pool *p, int32_t steps) { return pool_launch_impl(p, steps); }
+
+int nsl_pool_ready(nsl_pool *p) {
+  if (!p) return fail(NSL_ERR_ARG, "null pool");
+  const hipError_t e = hipStreamQuery(p->st);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); return 0; }
+  return fail(NSL_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
 
 int nsl_pool_collect(nsl_pool *p, nsl_steprec *recs, int32_t recs_per_slot, int64_t *n_done, int32_t *n_running) {
   int nr = 0;

@@ -64,7 +64,7 @@ class Spectral {
 }
 
 // ------------------------------------------------------------------------------------------------ solver
-function createSolver(opts) {
+function createSolver(opts, pool) {
   const N = opts.N || 32, NH = N / 2 + 1, NS = N * N * NH, N3 = N * N * N;
   const nu = opts.nu != null ? opts.nu : 1 / (opts.Re || 100), Re = 1 / nu;
   const cfl = opts.cfl || 0.4, dtFixed = opts.dt || 0;
@@ -72,15 +72,19 @@ function createSolver(opts) {
   const kc = Math.floor(N / 3);
   const ic0 = opts.ic || 'tgv';
   const v0 = icVector(ic0, opts.icParams);
-  const h = addon.create(N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3]);
+  // a pool slot (Pool.admit) or a solve of its own
+  let h, slot = -1;
+  if (pool) [slot, h] = addon.poolAdmit(pool.h, N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3]);
+  else h = addon.create(N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3]);
   const st = { t: 0, step: 0, dt: 0, N, Re, nu, kc, ic: ic0, icParams: opts.icParams || {}, version: VERSION,
     series: { t: [], E: [], Z: [], eps: [], omMax: [], uMax: [], dt: [], ebal: [], zbal: [], Pspec: [], Pal: [] },
     maxEbal: 0, maxZbal: 0, maxTnl: 0, divMax: 0, outputs: [] };
   let last = null;
   // device clock as last pushed, so assignments to st (checkpoint restore) reach the device before stepping
   let devClock = { t: 0, step: 0, E0: 0 };
-  // host mirror of the state: handed out by the _S getter, uploaded before the next device operation
-  const S = [0, 1, 2, 3, 4, 5].map(() => new Float64Array(NS));
+  // host mirror of the state (allocated on first use): handed out by the _S getter, uploaded before the next device
+  // operation
+  let S = null;
   let sOut = false;
   const sp = new Spectral(N);
 
@@ -116,6 +120,21 @@ function createSolver(opts) {
     st.ic = kind; st.icParams = prm;
     return finishIC();
   }
+  // Books `got` step records (21 values each: t dt step ebal zbal last[8] d0[8]) starting at r[base] as step()
+  // leaves st (nslab.js:314-332).
+  function book(r, base, got) {
+    for (let i = 0; i < got; i++) {
+      const o = base + i * 21;
+      const d0 = toDiag(r.subarray(o + 13, o + 21));
+      st.maxTnl = Math.max(st.maxTnl, Math.abs(d0.Tnl) / Math.max(d0.eps, 1e-300));
+      st.t = r[o]; st.dt = r[o + 1]; st.step = r[o + 2];
+      const ebal = r[o + 3], zbal = r[o + 4];
+      st.maxEbal = Math.max(st.maxEbal, ebal); st.maxZbal = Math.max(st.maxZbal, zbal);
+      last = toDiag(r.subarray(o + 5, o + 13));
+      record(st.dt); st.series.ebal.push(ebal); st.series.zbal.push(zbal);
+    }
+    if (got > 0) devClock = { t: st.t, step: st.step, E0: st.E0 };
+  }
   /** run(nSteps, tEnd) (nslab.js:334): steps on the device in chunks; the series are booked as record() does. */
   function run(nSteps, tEnd) {
     sync();
@@ -124,17 +143,7 @@ function createSolver(opts) {
       const k = Math.min(left, 1024);
       const r = addon.run(h, k, tEnd != null ? 1 : 0, tEnd != null ? tEnd : 0);
       const got = r.length / 21;
-      for (let i = 0; i < got; i++) {
-        const o = i * 21;
-        const d0 = toDiag(r.subarray(o + 13, o + 21));
-        st.maxTnl = Math.max(st.maxTnl, Math.abs(d0.Tnl) / Math.max(d0.eps, 1e-300));
-        st.t = r[o]; st.dt = r[o + 1]; st.step = r[o + 2];
-        const ebal = r[o + 3], zbal = r[o + 4];
-        st.maxEbal = Math.max(st.maxEbal, ebal); st.maxZbal = Math.max(st.maxZbal, zbal);
-        last = toDiag(r.subarray(o + 5, o + 13));
-        record(st.dt); st.series.ebal.push(ebal); st.series.zbal.push(zbal);
-      }
-      devClock = { t: st.t, step: st.step, E0: st.E0 };
+      book(r, 0, got);
       left -= got;
       if (got < k) break;
     }
@@ -273,10 +282,12 @@ function createSolver(opts) {
   const solver = { st, get last() { return last; }, step, run, diagnose, spectrum, slice, exactError, health, dossier, setIC,
     norms, interpMax, imageDiag, fields, N, nu, Re, kc, _sp: sp,
     deviceBytes: () => addon.deviceBytes(h),
-    profile: on => addon.profile(h, on ? 1 : 0), profileRead: () => addon.profileRead(h) };
+    profile: on => addon.profile(h, on ? 1 : 0), profileRead: () => addon.profileRead(h),
+    _slot: slot, _book: book, _sync: sync };
   /** _S: the six state planes (nslab.js:184), copied from the device on access; writes to them reach the device
    *  before the next solver operation. */
   Object.defineProperty(solver, '_S', { enumerable: true, get() {
+    if (!S) S = [0, 1, 2, 3, 4, 5].map(() => new Float64Array(NS));
     if (!sOut) { addon.getState(h, S, NS); sOut = true; }
     return S;
   } });
@@ -300,4 +311,43 @@ function studySummary(levels, key) {
   return { rows, verdict, text, order };
 }
 
-module.exports = { VERSION, IC_INFO, createSolver, studySummary, Spectral, fftPlan, fft1, factorize, addonVersion: addon.version() };
+// ------------------------------------------------------------------------------------------------ pools
+/**
+ * Solves of one N resident on the device and stepped together: one launch per pass covers every slot, one RK4
+ * step of the pool is a captured hipGraph. admit() returns a solver object with the createSolver() surface; its
+ * series are booked by collect(). A slot's results are bit-identical to the same solve run alone.
+ */
+class Pool {
+  constructor(N, slots) {
+    this.h = addon.poolCreate(N, slots);
+    this.N = N; this.slots = slots;
+    this.members = new Map();   // slot -> solver
+  }
+  static slotBytes(N) { return addon.poolSlotBytes(N); }
+  info() { const a = addon.poolInfo(this.h); return { N: a[0], slots: a[1], used: a[2], graphed: a[3], bytes: a[4] }; }
+  get free() { return this.slots - this.members.size; }
+  /** createSolver(opts) into a free slot (throws "pool is full" when none is). */
+  admit(opts) {
+    const s = createSolver(Object.assign({}, opts, { N: this.N }), this);
+    this.members.set(s._slot, s);
+    return s;
+  }
+  /** The solver's next launches run up to n steps, stopping at tEnd (run(n, tEnd) semantics). */
+  setTarget(s, n, tEnd) { s._sync(); addon.poolSetTarget(this.h, s._slot, n, tEnd != null ? 1 : 0, tEnd != null ? tEnd : 0); }
+  /** Enqueue up to k (<= 64) steps of every slot with steps left; returns at once. */
+  launch(k) { this.k = k || 64; addon.poolLaunch(this.h, this.k); }
+  /** True once the launched steps have finished (collect() will not wait). */
+  ready() { return addon.poolReady(this.h); }
+  /** Wait for the launched steps, book them into each member's series; returns the number of slots still running. */
+  collect() {
+    const k = this.k || 64, r = addon.poolCollect(this.h, k), head = 1 + this.slots;
+    for (const [slot, s] of this.members) { const got = r[1 + slot]; if (got > 0) s._book(r, head + slot * k * 21, got); }
+    return r[0];
+  }
+  release(s) { addon.poolRelease(this.h, s._slot); this.members.delete(s._slot); }
+}
+
+function deviceMemory() { const a = addon.deviceMemory(); return { free: a[0], total: a[1] }; }
+
+module.exports = { VERSION, IC_INFO, createSolver, studySummary, Spectral, fftPlan, fft1, factorize, Pool, deviceMemory,
+  addonVersion: addon.version() };

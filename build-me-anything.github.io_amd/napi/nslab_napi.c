@@ -374,6 +374,185 @@ static napi_value js_profile_read(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* ------------------------------------------------------------------------------------------------ pools */
+static void finalize_pool(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  nsl_pool_destroy((nsl_pool *)data);
+}
+
+static nsl_pool *get_pool(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  return (nsl_pool *)p;
+}
+
+#define POOL(idx)                                                           \
+  nsl_pool *pl = get_pool(env, argv[idx]);                                  \
+  if (!pl) { napi_throw_type_error(env, NULL, "not a pool handle"); return NULL; }
+
+static napi_value js_pool_slot_bytes(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  int64_t N = 0;
+  get_i64(env, argv[0], &N);
+  const int64_t b = nsl_pool_slot_bytes((int32_t)N);
+  if (b < 0) return throw_status(env, (int)b);
+  napi_value v;
+  CALL(env, napi_create_double(env, (double)b, &v));
+  return v;
+}
+
+/* poolCreate(N, slots) -> pool handle (destroyed when collected) */
+static napi_value js_pool_create(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  int64_t N = 0, slots = 0;
+  get_i64(env, argv[0], &N);
+  get_i64(env, argv[1], &slots);
+  nsl_pool *p = NULL;
+  int rc = nsl_pool_create((int32_t)N, (int32_t)slots, &p);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value ext;
+  CALL(env, napi_create_external(env, p, finalize_pool, NULL, &ext));
+  return ext;
+}
+
+/* poolInfo(p) -> [N, slots, used, graphed, bytes] */
+static napi_value js_pool_info(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  POOL(0);
+  int32_t N, slots, used, graphed;
+  int64_t bytes;
+  int rc = nsl_pool_info(pl, &N, &slots, &used, &graphed, &bytes);
+  if (rc < 0) return throw_status(env, rc);
+  double *d;
+  napi_value out = new_f64arr(env, 5, &d);
+  d[0] = N; d[1] = slots; d[2] = used; d[3] = graphed; d[4] = (double)bytes;
+  return out;
+}
+
+/* poolAdmit(p, N, ic, nu, cfl, dt, icp0..icp3) -> [slot, solver handle]. The handle is the slot's view: it has no
+ * finaliser (nsl_destroy ignores views); the slot is freed by poolRelease and the memory with the pool. */
+static napi_value js_pool_admit(napi_env env, napi_callback_info info) {
+  ARGS(10);
+  POOL(0);
+  nsl_opts o;
+  memset(&o, 0, sizeof o);
+  int64_t N = 0, ic = 0;
+  if (argc < 10 || !get_i64(env, argv[1], &N) || !get_i64(env, argv[2], &ic) || !get_f64(env, argv[3], &o.nu) ||
+      !get_f64(env, argv[4], &o.cfl) || !get_f64(env, argv[5], &o.dt)) {
+    napi_throw_type_error(env, NULL, "poolAdmit(p, N, ic, nu, cfl, dt, p0, p1, p2, p3)");
+    return NULL;
+  }
+  for (int i = 0; i < 4; i++) if (!get_f64(env, argv[6 + i], &o.icp[i])) o.icp[i] = 0;
+  o.N = (int32_t)N;
+  o.ic = (int32_t)ic;
+  int32_t slot = -1;
+  int rc = nsl_pool_admit(pl, &o, &slot);
+  if (rc < 0) return throw_status(env, rc);
+  nsl_solver *v = NULL;
+  rc = nsl_pool_slot(pl, slot, &v);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value arr, sv, ext;
+  CALL(env, napi_create_array_with_length(env, 2, &arr));
+  CALL(env, napi_create_int32(env, slot, &sv));
+  CALL(env, napi_create_external(env, v, NULL, NULL, &ext));
+  CALL(env, napi_set_element(env, arr, 0, sv));
+  CALL(env, napi_set_element(env, arr, 1, ext));
+  return arr;
+}
+
+static napi_value js_pool_set_target(napi_env env, napi_callback_info info) {
+  ARGS(5);
+  POOL(0);
+  int64_t slot = 0, n = 0, has = 0;
+  double tend = 0;
+  get_i64(env, argv[1], &slot);
+  get_i64(env, argv[2], &n);
+  get_i64(env, argv[3], &has);
+  get_f64(env, argv[4], &tend);
+  int rc = nsl_pool_set_target(pl, (int32_t)slot, n, (int32_t)has, tend);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+static napi_value js_pool_launch(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  POOL(0);
+  int64_t k = 0;
+  get_i64(env, argv[1], &k);
+  int rc = nsl_pool_launch(pl, (int32_t)k);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+/* poolCollect(p, k) -> Float64Array: [running, done[0..slots), then slots*k records of 21 values
+ * (t dt step ebal zbal last[8] d0[8]); slot i's records start at 1 + slots + i*k*21] */
+static napi_value js_pool_collect(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  POOL(0);
+  int64_t k = 0;
+  get_i64(env, argv[1], &k);
+  int32_t N, slots, used, graphed;
+  int64_t bytes;
+  int rc = nsl_pool_info(pl, &N, &slots, &used, &graphed, &bytes);
+  if (rc < 0) return throw_status(env, rc);
+  if (k < 1) k = 1;
+  nsl_steprec *recs = (nsl_steprec *)calloc((size_t)slots * (size_t)k, sizeof(nsl_steprec));
+  int64_t *done = (int64_t *)calloc((size_t)slots, sizeof(int64_t));
+  if (!recs || !done) { free(recs); free(done); napi_throw_error(env, NULL, "out of host memory"); return NULL; }
+  int32_t running = 0;
+  rc = nsl_pool_collect(pl, recs, (int32_t)k, done, &running);
+  if (rc < 0) { free(recs); free(done); return throw_status(env, rc); }
+  double *d;
+  const size_t head = 1 + (size_t)slots;
+  napi_value out = new_f64arr(env, head + (size_t)slots * (size_t)k * 21, &d);
+  d[0] = running;
+  for (int32_t i = 0; i < slots; i++) {
+    d[1 + i] = (double)done[i];
+    for (int64_t j = 0; j < done[i] && j < k; j++) {
+      const nsl_steprec *r = &recs[(size_t)i * (size_t)k + (size_t)j];
+      double *o = d + head + ((size_t)i * (size_t)k + (size_t)j) * 21;
+      o[0] = r->t; o[1] = r->dt; o[2] = (double)r->step; o[3] = r->ebal; o[4] = r->zbal;
+      memcpy(o + 5, &r->last, 8 * sizeof(double));
+      memcpy(o + 13, &r->d0, 8 * sizeof(double));
+    }
+  }
+  free(recs);
+  free(done);
+  return out;
+}
+
+static napi_value js_pool_ready(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  POOL(0);
+  int rc = nsl_pool_ready(pl);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value v;
+  CALL(env, napi_get_boolean(env, rc == 1, &v));
+  return v;
+}
+
+/* deviceMemory() -> [free, total] bytes */
+static napi_value js_device_memory(napi_env env, napi_callback_info info) {
+  (void)info;
+  int64_t f = 0, t = 0;
+  int rc = nsl_device_memory(&f, &t);
+  if (rc < 0) return throw_status(env, rc);
+  double *d;
+  napi_value out = new_f64arr(env, 2, &d);
+  d[0] = (double)f; d[1] = (double)t;
+  return out;
+}
+
+static napi_value js_pool_release(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  POOL(0);
+  int64_t slot = 0;
+  get_i64(env, argv[1], &slot);
+  int rc = nsl_pool_release(pl, (int32_t)slot);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -390,7 +569,10 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"exactError", js_exact_error}, {"deviceBytes", js_device_bytes}, {"fft1", js_fft1},
     {"spectralForward", js_spectral_forward}, {"spectralInverse", js_spectral_inverse},
     {"factorize", js_factorize}, {"profile", js_profile}, {"profileRead", js_profile_read},
-    {"version", js_version},
+    {"version", js_version}, {"poolSlotBytes", js_pool_slot_bytes}, {"poolCreate", js_pool_create},
+    {"poolInfo", js_pool_info}, {"poolAdmit", js_pool_admit}, {"poolSetTarget", js_pool_set_target},
+    {"poolLaunch", js_pool_launch}, {"poolCollect", js_pool_collect}, {"poolRelease", js_pool_release},
+    {"deviceMemory", js_device_memory}, {"poolReady", js_pool_ready},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

@@ -142,11 +142,14 @@ int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count);
  *                                  (state, clock, last, diagnose, spectrum, nsl_run); nsl_destroy ignores it
  *   nsl_pool_set_target(p, i, n, has_tend, tend)   the slot runs n more steps (stopping at tend), as run(n, tEnd)
  *   nsl_pool_launch(p, k)          enqueues k <= 64 steps of every slot with steps left; returns at once
+ *   nsl_pool_ready(p)              1 once the launched steps have finished (collect will not wait), else 0
  *   nsl_pool_collect(p, recs, cap, n_done, &running)   waits; per slot i: n_done[i] steps and their records at
  *                                  recs[i*cap ..] (recs may be null); running = slots with steps left
  *   nsl_pool_release(p, i)         frees the slot */
 typedef struct nsl_pool nsl_pool;
 int64_t nsl_pool_slot_bytes(int32_t N);
+/* Free and total device memory of the current GPU (the admission budget of a request loop). */
+int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out);
 void nsl_pool_destroy(nsl_pool *p);
 int nsl_pool_info(nsl_pool *p, int32_t *N, int32_t *slots, int32_t *used, int32_t *graphed, int64_t *bytes);
@@ -154,6 +157,7 @@ int nsl_pool_admit(nsl_pool *p, const nsl_opts *opts, int32_t *slot);
 int nsl_pool_slot(nsl_pool *p, int32_t slot, nsl_solver **view);
 int nsl_pool_set_target(nsl_pool *p, int32_t slot, int64_t n, int32_t has_tend, double tend);
 int nsl_pool_launch(nsl_pool *p, int32_t steps);
+int nsl_pool_ready(nsl_pool *p);
 int nsl_pool_collect(nsl_pool *p, nsl_steprec *recs, int32_t recs_per_slot, int64_t *n_done, int32_t *n_running);
 int nsl_pool_release(nsl_pool *p, int32_t slot);
 

@@ -84,6 +84,7 @@ def lib():
         L.nsl_pool_slot.argtypes = [vp, ctypes.c_int32, pp]
         L.nsl_pool_set_target.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_double]
         L.nsl_pool_launch.argtypes = [vp, ctypes.c_int32]
+        L.nsl_pool_ready.argtypes = [vp]
         L.nsl_pool_collect.argtypes = [vp, ctypes.POINTER(StepRec), ctypes.c_int32, i64p, i32p]
         L.nsl_pool_release.argtypes = [vp, ctypes.c_int32]
         L.nsl_last_error.restype = ctypes.c_char_p

@@ -37,3 +37,12 @@ def test_validate_nslab_on_gpu(group):
     r = subprocess.run([node, SCRIPT, group], capture_output=True, text=True, timeout=600)
     print(r.stdout)
     assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
+@pytest.mark.gpu
+def test_pools_and_request_loop_on_gpu():
+    script = os.path.join(ROOT, "tests", "js", "validate_pools.js")
+    r = subprocess.run([node, script], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
