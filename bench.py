@@ -7,10 +7,14 @@ resident in HBM when the timed region starts; the timed region is K steps betwee
 
 Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
 its mean launch time, from HIP events recorded on the solver's stream inside the timed region), the whole-step
-roofline fraction of the truncated-storage traffic model, a 512^3 line, and the CPU baseline (the oracle, a
-single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample).
+roofline fraction of the truncated-storage traffic model, a 512^3 line, the CPU baseline (the oracle, a
+single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and two lines from
+the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step) and the
+ladder request loop (Poisson arrivals, admission by HBM footprint, p50/p99 latency, ladders per hour, against
+sequential single-solve execution of the same work).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256] [--no-512] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256] [--no-512] [--no-cpu] [--no-ensemble]
+                  [--no-serve] [--serve-requests R]
 """
 import argparse
 import ctypes
@@ -113,6 +117,23 @@ def cpu_baseline(N, dt, budget_s=20.0):
                       "thread; host %s, %d cores visible" % (N, dt, n, t_setup, el, _cpu_model(), os.cpu_count())}
 
 
+def node_leg(args, timeout):
+    """Runs js/serve.js under Node with `args`; returns its JSON object, or {"error": ...}."""
+    import shutil
+    import subprocess
+    node = shutil.which("node")
+    if node is None:
+        return {"error": "node is not installed"}
+    script = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "serve.js")
+    try:
+        r = subprocess.run([node, script] + args, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after %d s" % timeout}
+    if r.returncode != 0:
+        return {"error": (r.stderr or r.stdout)[-800:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -144,6 +165,9 @@ def main():
     ap.add_argument("--dt", type=float, default=0.004)
     ap.add_argument("--no-512", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device-to-device copy of this many MiB first (PMC counter calibration)")
     a = ap.parse_args()
@@ -200,6 +224,30 @@ def main():
                        "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
                                    for n, v in extra["kernels"].items()}}
+    if not a.no_ensemble:
+        e = node_leg(["ensemble", "--steps", "300", "--seq-per-n", "4"], 600)
+        out["ensemble"] = e if "error" in e else {
+            "workload": "128 solves in slot pools: 64x32^3, 32x48^3, 24x64^3, 8x96^3, random IC k0=4 E0=0.5, "
+                        "Re log-uniform [707,4000], CFL 0.5, 300 RK4 steps, one hipGraph per pool step",
+            "solve_steps_per_s": e["solve_steps_per_s"], "run_s": e["run_s"], "admit_s": e["admit_s"],
+            "sequential_s": e.get("sequential_s"), "speedup_vs_sequential": e.get("speedup_vs_sequential"),
+            "pool_equals_single": e.get("pool_equals_single"), "graphed": e["graphed"],
+            "b_alg_GBs": sum(n * b_alg_step(N) for N, n in ((32, 64), (48, 32), (64, 24), (96, 8))) * 300
+                         / e["run_s"] / 1e9}
+    if not a.no_serve:
+        sv = node_leg(["serve", "--requests", str(a.serve_requests), "--rate", "2", "--seed", "1",
+                       "--seq-sample", "2"], 900)
+        out["serving"] = sv if "error" in sv else {
+            "workload": "%d ladder requests, Poisson 2/s, ladders (48,64,96)/(64,96,128)/(96,128,192)/(128,192,256), "
+                        "tubes or random IC, Re log-uniform [707,4000], 200-1000 steps on the finest rung, CFL 0.5"
+                        % sv["requests"],
+            "ladders_per_hour": sv["ladders_per_hour"], "rung_solves_per_hour": sv["rung_solves_per_hour"],
+            "p50_latency_s": sv["p50_latency_s"], "p99_latency_s": sv["p99_latency_s"], "wall_s": sv["wall_s"],
+            "sequential_estimate_s": sv["sequential_estimate_s"],
+            "throughput_vs_sequential": sv["throughput_vs_sequential"],
+            "sequential_sample_s": sv.get("sequential_sample_s"),
+            "sequential_sample_estimate_s": sv.get("sequential_sample_estimate_s"),
+            "served_equals_sequential": sv.get("served_equals_sequential")}
     print(json.dumps(out))
 
 
