@@ -63,15 +63,20 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restric
 // IFFT_z(omega-hat_z) = i (kx A_y - ky A_x). Type 0 transforms u_x, u_y and writes u_x, u_y, omega_z; type 1
 // transforms u_z, i kz u_y, i kz u_x and writes u_z, omega_x, omega_y: five transforms and five state-tile reads per
 // tile instead of six and nine. Grid ntile * 2 * by, XCD-remapped so the pair shares an L2.
+// Waves per SIMD the z-inverse must fit: at N >= 512 (one butterfly per thread) a cap of 128 registers costs no
+// spills and lets two workgroups share a CU.
+template <int N> struct ZinvMinWaves { static constexpr int v = N >= 512 ? 4 : 1; };
+// Threads of the z-inverse: at N = 256 (16 x 16, 16 outputs per butterfly) 128 threads, one butterfly each.
+template <int N> struct ZinvNT { static constexpr int v = N == 256 ? 128 : PassNT<N>::v; };
 template <int N, int T>
-__global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+__global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
                                                       const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
   U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
-  using OR = OutRegs<N, T, PassNT<N>::v>;
+  using OR = OutRegs<N, T, ZinvNT<N>::v>;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = bid % ntile, rest = bid / ntile, type = rest & 1, cy = rest >> 1, kx0 = tile * T;
   const double ky = kwv(full_of(cy, g), N);
@@ -101,10 +106,10 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__rest
   };
   if (type == 0) {
     auto lx = load_c(0, false);
-    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lx, keepA);
+    fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, lx, keepA);
     out(0, [&](int k, int j) { return A.v[k][j]; });
     auto ly = load_c(1, false);
-    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, ly, keepB);
+    fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, ly, keepB);
     out(1, [&](int k, int j) { return B.v[k][j]; });
     out(5, [&](int k, int j) {
       const double kx = (double)(kx0 + OR::line(k));
@@ -113,15 +118,15 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv_uw(const double2 *__rest
     });
   } else {
     auto lz = load_c(2, false);
-    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lz, keepA);
+    fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, lz, keepA);
     out(2, [&](int k, int j) { return A.v[k][j]; });
     auto ly = load_c(1, true);
-    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, ly, keepB);
+    fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, ly, keepB);
     out(3, [&](int k, int j) {
       return make_double2(-ky * A.v[k][j].y - B.v[k][j].x, ky * A.v[k][j].x - B.v[k][j].y);
     });
     auto lx = load_c(0, true);
-    fft<N, T, PassNT<N>::v, 1, false>(lds, tw, lx, keepB);
+    fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, lx, keepB);
     out(4, [&](int k, int j) {
       const double kx = (double)(kx0 + OR::line(k));
       return make_double2(B.v[k][j].x + kx * A.v[k][j].y, B.v[k][j].y - kx * A.v[k][j].x);
