@@ -97,6 +97,27 @@ def measure(N, steps, warmup, dt, label):
             "step_roofline_frac": b_alg_step(N) * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
 
 
+def measure_tubes(N, steps):
+    """The memory-heavy configuration: antiparallel tubes along z (Gamma = 1, sigma = 0.3, centres pi +- (0.8 +
+    0.1 cos z), seeded perturbation rms 1e-4), Re = Gamma/nu = 2000, CFL 0.5; `steps` RK4 steps timed after one."""
+    import capi
+    import torch
+    t0 = time.perf_counter()
+    s = capi.Solver(N=N, Re=2000, ic="tubesz", cfl=0.5)
+    setup = time.perf_counter() - t0
+    s.run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = s.run(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    r = {"workload": "tube pair along z %d^3, Re 2000, CFL 0.5, FP64" % N, "steps": done, "steps_per_s": done / el,
+         "step_roofline_frac": b_alg_step(N) * done / el / (HBM_PEAK_GBS * 1e9), "setup_s": setup,
+         "device_GB": s.device_bytes() / 1e9, "t": s.clock["t"], "omMax": s.last["omMax"]}
+    s.close()
+    return r
+
+
 def cpu_baseline(N, dt, budget_s=20.0):
     """The oracle (single-threaded C restatement of the reference algorithm, 5 rhs per step as nslab.js) timed on
     this host: one untimed set-up, then RK4 steps until about budget_s seconds of CPU work (at most 3)."""
@@ -165,6 +186,7 @@ def main():
     ap.add_argument("--dt", type=float, default=0.004)
     ap.add_argument("--no-512", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tubes", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
@@ -224,6 +246,8 @@ def main():
                        "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
                                    for n, v in extra["kernels"].items()}}
+    if not a.no_tubes:
+        out["tubes"] = [measure_tubes(512, 10), measure_tubes(1024, 3)]
     if not a.no_ensemble:
         e = node_leg(["ensemble", "--steps", "300", "--seq-per-n", "4"], 600)
         out["ensemble"] = e if "error" in e else {

@@ -483,7 +483,7 @@ __global__ void k_chunk_begin(Ctl *ctl, size_t sb) { at_slot(ctl, sb)->nChunk = 
 
 // ------------------------------------------------------------------------------------------------ set-up kernels
 // Physical initial fields (nslab.js:251-266): kind 0 tgv, 1 tgv2d, 2 abc into phys[0..2] (u, v, w);
-// kind 3 tubes: omega_x into phys[0], phys[1..2] = 0.
+// kind 3 tubes: omega_x into phys[0], phys[1..2] = 0; kind 4 (tube pair along z): omega into phys[0..2].
 __global__ void k_ic_phys(double *__restrict__ phys, int N, int kind, double p0, double p1, double p2, double p3) {
   const size_t N3 = (size_t)N * N * N;
   const double h = NSL_TWO_PI / N, PI = 3.141592653589793;
@@ -494,7 +494,25 @@ __global__ void k_ic_phys(double *__restrict__ phys, int N, int kind, double p0,
     if (kind == 0) { u = sin(x) * cos(y) * cos(z); v = -cos(x) * sin(y) * cos(z); w = 0; }
     else if (kind == 1) { u = sin(x) * cos(y); v = -cos(x) * sin(y); w = 0; }
     else if (kind == 2) { u = p0 * sin(z) + p2 * cos(y); v = p1 * sin(x) + p0 * cos(z); w = p2 * sin(y) + p1 * cos(x); }
-    else {
+    else if (kind == 4) {
+      // two Gaussian-core tubes along z, circulation +-p0, core radius p1, centres (pi +- (p2 + 0.1 cos z), pi);
+      // omega along the centreline tangent (dx_c/dz, 0, 1), nearest periodic image over a 3x3 lattice
+      const double G = p0, sig = p1, d0 = p2, a = 0.1;
+      double wx = 0, wz = 0;
+      for (int sgn = -1; sgn <= 1; sgn += 2) {
+        const double xc = PI + sgn * (d0 + a * cos(z)), yc = PI;
+        double best = INFINITY;
+        for (int px = -1; px <= 1; px++)
+          for (int py = -1; py <= 1; py++) {
+            const double dx = x - xc + px * NSL_TWO_PI, dy = y - yc + py * NSL_TWO_PI;
+            best = fmin(best, dx * dx + dy * dy);
+          }
+        const double om = sgn * G / (PI * sig * sig) * exp(-best / (sig * sig));
+        wz += om;
+        wx += -sgn * a * sin(z) * om;
+      }
+      u = wx; v = 0; w = wz;
+    } else {
       const double amp = p0, sig = p1, d0 = p2, del = p3;
       const double d = d0 + del * cos(x);
       double s = 0;
@@ -553,6 +571,14 @@ __global__ void k_tubes_finish(const double2 *__restrict__ W, double2 *__restric
     u0.x /= kd; u0.y /= kd; u1.x /= kd; u1.y /= kd; u2.x /= kd; u2.y /= kd;
     project_mode(kx, ky, kz, k2, u0, u1, u2);
     S[m] = u0; S[D + m] = u1; S[2 * D + m] = u2;
+  }
+}
+
+// S += T over n complex values (a perturbation added to an initial field).
+__global__ void k_axpy(double2 *__restrict__ S, const double2 *__restrict__ T, long long n) {
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long long)gridDim.x * blockDim.x) {
+    const double2 a = S[m], b = T[m];
+    S[m] = make_double2(a.x + b.x, a.y + b.y);
   }
 }
 

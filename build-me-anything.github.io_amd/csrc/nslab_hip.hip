@@ -526,11 +526,10 @@ static inline double kw_h(int f, int N) { return (double)(f <= N / 2 ? f : f - N
 // Random solenoidal field (nslab.js:273-286): the PRNG draws follow the reference's loop order over the masked
 // modes, which is the compact storage order. Drawn on the host (a sequential stream), then projected and scaled
 // to E0 on the device.
-static int ic_random(nsl_solver *s) {
+static int ic_random(nsl_solver *s, double2 *dst, double seed, double k0, double E0) {
   const Geo &g = s->x.g;
   const int N = g.N;
-  uint32_t a = (uint32_t)(int32_t)s->icp[0];
-  const double k0 = s->icp[1], E0 = s->icp[2];
+  uint32_t a = (uint32_t)(int32_t)seed;
   std::vector<double2> h((size_t)3 * g.D, make_double2(0.0, 0.0));
   for (int cz = 0; cz < g.by; cz++) {
     const int l = full_of_h(cz, g);
@@ -566,34 +565,41 @@ static int ic_random(nsl_solver *s) {
       }
     }
   }
-  HIP_TRY(hipMemcpyAsync(s->S, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s->st));
-  hipLaunchKernelGGL(k_project, dim3(sweep_grid(g.D)), dim3(NSL_NT), 0, s->st, s->S, g);
+  HIP_TRY(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s->st));
+  hipLaunchKernelGGL(k_project, dim3(sweep_grid(g.D)), dim3(NSL_NT), 0, s->st, dst, g);
   LAUNCH_CHECK();
   double *nrm = (double *)((char *)s->ctl + offsetof(Ctl, nrm));   // device scratch for the three norms
-  TRY(norms_dev(s, s->S, nrm));
-  hipLaunchKernelGGL(k_scale_energy, dim3(sweep_grid(3 * g.D)), dim3(NSL_NT), 0, s->st, s->S, 3 * g.D, (const double *)nrm, E0);
+  TRY(norms_dev(s, dst, nrm));
+  hipLaunchKernelGGL(k_scale_energy, dim3(sweep_grid(3 * g.D)), dim3(NSL_NT), 0, s->st, dst, 3 * g.D, (const double *)nrm, E0);
   LAUNCH_CHECK();
   HIP_TRY(hipStreamSynchronize(s->st));
   return NSL_OK;
 }
 
 static int set_ic(nsl_solver *s, int ic, const double *icp) {
-  if (ic < 0 || ic > 4) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(ic));
+  if (ic < 0 || ic > NSL_IC_TUBESZ) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(ic));
   s->ic = ic;
   for (int q = 0; q < 4; q++) s->icp[q] = icp ? icp[q] : 0;
   const Geo &g = s->x.g;
   const int N = g.N;
   if (ic == NSL_IC_RANDOM) {
-    TRY(ic_random(s));
+    TRY(ic_random(s, s->S, s->icp[0], s->icp[1], s->icp[2]));
   } else {
     double *phys = (double *)s->X2;
-    hipLaunchKernelGGL(k_ic_phys, dim3(sweep_grid((long long)N * N * N)), dim3(NSL_NT), 0, s->st, phys, N, ic == NSL_IC_TUBES ? 3 : ic,
+    hipLaunchKernelGGL(k_ic_phys, dim3(sweep_grid((long long)N * N * N)), dim3(NSL_NT), 0, s->st, phys, N, ic == NSL_IC_TUBES ? 3 : ic == NSL_IC_TUBESZ ? 4 : ic,
                        s->icp[0], s->icp[1], s->icp[2], s->icp[3]);
     LAUNCH_CHECK();
-    if (ic == NSL_IC_TUBES) {
+    if (ic == NSL_IC_TUBES || ic == NSL_IC_TUBESZ) {
       TRY(forward3_from_phys(s, s->NL));
       hipLaunchKernelGGL(k_tubes_finish, dim3(sweep_grid(g.D)), dim3(NSL_NT), 0, s->st, s->NL, s->S, g);
       LAUNCH_CHECK();
+      if (ic == NSL_IC_TUBESZ && s->icp[3] > 0) {
+        // seeded random solenoidal perturbation (k^4 exp(-2 (k/4)^2), seed 1), rms velocity icp[3] per component:
+        // energy 3/2 icp[3]^2
+        TRY(ic_random(s, s->T, 1.0, 4.0, 1.5 * s->icp[3] * s->icp[3]));
+        hipLaunchKernelGGL(k_axpy, dim3(sweep_grid(3 * g.D)), dim3(NSL_NT), 0, s->st, s->S, (const double2 *)s->T, 3 * g.D);
+        LAUNCH_CHECK();
+      }
     } else {
       TRY(forward3_from_phys(s, s->S));
       hipLaunchKernelGGL(k_project, dim3(sweep_grid(g.D)), dim3(NSL_NT), 0, s->st, s->S, g);
@@ -647,7 +653,7 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
   std::vector<int> f;
   if (factorize_host(N, f) < 0) return fail(NSL_ERR_ARG, bad_n_msg(N));
   if (N % 2) return fail(NSL_ERR_ARG, "grid size must be even for the real-to-complex transforms (got " + std::to_string(N) + ")");
-  if (o->ic < 0 || o->ic > 4) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
+  if (o->ic < 0 || o->ic > NSL_IC_TUBESZ) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
   nsl_solver *s = new nsl_solver();
   s->N = N; s->nu = o->nu; s->cfl = o->cfl; s->dtFixed = o->dt;
   hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
@@ -847,7 +853,7 @@ static int pool_slot_check(nsl_pool *p, int slot, bool must_be_used) {
 static int pool_admit_impl(nsl_pool *p, const nsl_opts *o, int *slot) {
   if (!p || !o || !slot) return fail(NSL_ERR_ARG, "null argument");
   if (o->N != p->N) return fail(NSL_ERR_ARG, "grid size " + std::to_string(o->N) + " does not match the pool's " + std::to_string(p->N));
-  if (o->ic < 0 || o->ic > 4) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
+  if (o->ic < 0 || o->ic > NSL_IC_TUBESZ) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
   int i = 0;
   while (i < p->C && p->used[i]) i++;
   if (i == p->C) return fail(NSL_ERR_STATE, "pool is full");

@@ -14,7 +14,7 @@ const addon = require(path.join(__dirname, '..', 'lib', 'nslab_napi.node'));
 
 const VERSION = '0.1.2';
 const TWO_PI = 2 * Math.PI;
-const IC_KIND = { tgv: 0, tgv2d: 1, abc: 2, tubes: 3, random: 4 };
+const IC_KIND = { tgv: 0, tgv2d: 1, abc: 2, tubes: 3, random: 4, tubesz: 5 };
 
 const IC_INFO = {
   tgv: { name: 'Taylor–Green vortex (3D)', exact: null, desc: 'u = sin x cos y cos z, v = −cos x sin y cos z, w = 0 (Brachet et al. 1983)' },
@@ -22,6 +22,8 @@ const IC_INFO = {
   abc: { name: 'Arnold–Beltrami–Childress (exact decay)', exact: 'e^{-νt}', desc: 'u = A sin z + C cos y, v = B sin x + A cos z, w = C sin y + B cos x — Beltrami, exact solution ∝ e^{−νt}' },
   tubes: { name: 'Antiparallel vortex tubes', exact: null, desc: 'two Gaussian-core tubes of opposite circulation along x, sinusoidally perturbed (Kerr-type reconnection / stretching experiment)' },
   random: { name: 'Random solenoidal field', exact: null, desc: 'Gaussian random field with E(k) ∝ k⁴ exp(−2(k/k₀)²), seeded (decaying isotropic turbulence)' },
+  // not in the reference: the tube pair of the memory-heavy benchmark (Gaussian cores along z, velocity from vorticity)
+  tubesz: { name: 'Antiparallel vortex tubes along z', exact: null, desc: 'circulation ±Γ, core radius σ, centrelines (π ± (d + 0.1 cos z), π), plus a seeded solenoidal perturbation of rms velocity pert' },
 };
 
 // IC parameter vector with the reference's defaults (nslab.js:255-274).
@@ -31,6 +33,7 @@ function icVector(kind, prm) {
   if (!(kind in IC_KIND)) throw new Error('unknown initial condition ' + kind);
   if (kind === 'abc') return [nn(prm.A, 1), nn(prm.B, 1), nn(prm.C, 1), 0];
   if (kind === 'tubes') return [nn(prm.amp, 8), nn(prm.sigma, 0.4), nn(prm.sep, 0.7), nn(prm.pert, 0.2)];
+  if (kind === 'tubesz') return [nn(prm.Gamma, 1), nn(prm.sigma, 0.3), nn(prm.sep, 0.8), nn(prm.pert, 1e-4)];
   if (kind === 'random') return [(prm.seed || 1) | 0, prm.k0 || 4, prm.E0 || 0.5, 0];
   return [0, 0, 0, 0];
 }

@@ -30,6 +30,11 @@ extern "C" {
 #define NSL_IC_ABC 2
 #define NSL_IC_TUBES 3
 #define NSL_IC_RANDOM 4
+/* Antiparallel tube pair along z (no reference counterpart: the memory-heavy benchmark's initial field). icp =
+ * {Gamma, sigma, d, pert}: Gaussian cores of circulation +-Gamma and radius sigma, centrelines
+ * (pi +- (d + 0.1 cos z), pi); velocity from the vorticity spectrally (u = i k x omega / k^2), then a seeded random
+ * solenoidal perturbation (E(k) ~ k^4 exp(-2 (k/4)^2), seed 1) of rms velocity `pert` per component. */
+#define NSL_IC_TUBESZ 5
 
 typedef struct nsl_solver nsl_solver;
 

@@ -10,6 +10,9 @@
  *   solver set-up (kc, kx, kw, mask, hw) .. nslab.js:171-191
  *   project / curlHat / norms / rhs ....... nslab.js:194-247
  *   setIC (tgv, tgv2d, abc, tubes, random)  nslab.js:161, 250-302
+ *   tubesz (no reference counterpart) ..... the memory-heavy benchmark's tube pair along z: the tubes path of
+ *                                           nslab.js:257-272 on another vorticity field, plus the random field of
+ *                                           nslab.js:273-286 scaled to the perturbation energy
  *   chooseDt / step / run ................. nslab.js:305-334
  *   diagnose / eig3 / spectrum ............ nslab.js:339-420
  *   interpMax / imageDiag / cutoffPileUp .. nslab.js:421-564
@@ -228,7 +231,7 @@ int or_spectral_inverse(int N, const double *re, const double *im, double *f) {
 }
 
 /* ---------------------------------------------------------------- solver (nslab.js:171-334) */
-enum { IC_TGV = 0, IC_TGV2D = 1, IC_ABC = 2, IC_TUBES = 3, IC_RANDOM = 4 };
+enum { IC_TGV = 0, IC_TGV2D = 1, IC_ABC = 2, IC_TUBES = 3, IC_RANDOM = 4, IC_TUBESZ = 5 };
 
 typedef struct { double E, Z, P, eps, Tnl, Pspec, omMax, uMax; } or_diag;
 
@@ -340,6 +343,37 @@ static void finishIC(or_solver *s) {
   s->st.E0 = s->last.E; s->st.Z0 = s->last.Z;
 }
 
+/* Random solenoidal field (nslab.js:273-286) into dst: Mulberry32(seed) Box-Muller draws over the masked modes,
+ * Hermitian fill of kx = 0, project, scale to energy E0. */
+static void random_field(or_solver *s, double **dst, double seed, double k0, double E0) {
+  const int N = s->N, NH = s->NH;
+  mb32 g = { (uint32_t)(int32_t)seed };
+  for (int c = 0; c < 6; c++) memset(dst[c], 0, 8 * s->NS);
+  for (int l = 0; l < N; l++) { const double kz = s->kw[l];
+    for (int j = 0; j < N; j++) { const double ky = s->kw[j]; const long row = ((long)l * N + j) * NH;
+      for (int i = 0; i < NH; i++) {
+        const long idx = row + i; if (!s->mask[idx]) continue;
+        const double k2 = s->kx[i] * s->kx[i] + ky * ky + kz * kz; if (k2 == 0) continue;
+        if (i == 0 && (kz < 0 || (kz == 0 && ky < 0))) continue;
+        const double k = sqrt(k2), amp = sqrt(pow(k, 4) * exp(-2 * (k / k0) * (k / k0)) / (4 * M_PI * k2));
+        for (int c = 0; c < 3; c++) {
+          for (int part = 0; part < 2; part++) {
+            double a = mb32_next(&g); a = a > 1e-12 ? a : 1e-12;
+            const double b = mb32_next(&g);
+            dst[2 * c + part][idx] = amp * (sqrt(-2 * log(a)) * cos(TWO_PI * b));
+          }
+        }
+        if (i == 0) {
+          const int jm = (N - j) % N, lm = (N - l) % N; const long idm = ((long)lm * N + jm) * NH;
+          for (int c = 0; c < 3; c++) { dst[2 * c][idm] = dst[2 * c][idx]; dst[2 * c + 1][idm] = -dst[2 * c + 1][idx]; }
+        }
+      } } }
+  project(s, dst, 1);
+  double E, Z, P; norms(s, dst, &E, &Z, &P);
+  const double sc = sqrt(E0 / E);
+  for (int c = 0; c < 6; c++) for (long n = 0; n < s->NS; n++) dst[c][n] *= sc;
+}
+
 int or_setIC(or_solver *s, int kind, const double *prm) {
   const int N = s->N, NH = s->NH; const double h = TWO_PI / N;
   double *u = s->up[0], *v = s->up[1], *w = s->up[2];
@@ -354,21 +388,39 @@ int or_setIC(or_solver *s, int kind, const double *prm) {
     }
     for (int c = 0; c < 3; c++) spectral_forward(s->sp, s->up[c], s->S[2 * c], s->S[2 * c + 1]);
     project(s, s->S, 1);
-  } else if (kind == IC_TUBES) {
-    const double amp = prm[0], sig = prm[1], d0 = prm[2], del = prm[3];
+  } else if (kind == IC_TUBES || kind == IC_TUBESZ) {
     double *ox = s->om[0], *oy = s->om[1], *oz = s->om[2];
-    for (int l = 0; l < N; l++) for (int j = 0; j < N; j++) for (int i = 0; i < N; i++) {
-      const long n = ((long)l * N + j) * N + i; const double x = i * h, y = j * h, z = l * h;
-      const double d = d0 + del * cos(x); double sm = 0;
-      for (int sgn = -1; sgn <= 1; sgn += 2) {
-        const double yc = M_PI + sgn * d, zc = M_PI; double best = INFINITY;
-        for (int py = -1; py <= 1; py++) for (int pz = -1; pz <= 1; pz++) {
-          const double dy = y - yc + py * TWO_PI, dz = z - zc + pz * TWO_PI; const double q = dy * dy + dz * dz;
-          best = q < best ? q : best;
+    if (kind == IC_TUBES) {
+      const double amp = prm[0], sig = prm[1], d0 = prm[2], del = prm[3];
+      for (int l = 0; l < N; l++) for (int j = 0; j < N; j++) for (int i = 0; i < N; i++) {
+        const long n = ((long)l * N + j) * N + i; const double x = i * h, y = j * h, z = l * h;
+        const double d = d0 + del * cos(x); double sm = 0;
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          const double yc = M_PI + sgn * d, zc = M_PI; double best = INFINITY;
+          for (int py = -1; py <= 1; py++) for (int pz = -1; pz <= 1; pz++) {
+            const double dy = y - yc + py * TWO_PI, dz = z - zc + pz * TWO_PI; const double q = dy * dy + dz * dz;
+            best = q < best ? q : best;
+          }
+          sm += -sgn * amp * exp(-best / (sig * sig));
         }
-        sm += -sgn * amp * exp(-best / (sig * sig));
+        ox[n] = sm; oy[n] = 0; oz[n] = 0;
       }
-      ox[n] = sm; oy[n] = 0; oz[n] = 0;
+    } else {   /* tube pair along z: cores +-Gamma/(pi sigma^2) exp(-r^2/sigma^2), centres (pi +- (d + 0.1 cos z), pi) */
+      const double G = prm[0], sig = prm[1], d0 = prm[2], a = 0.1;
+      for (int l = 0; l < N; l++) for (int j = 0; j < N; j++) for (int i = 0; i < N; i++) {
+        const long n = ((long)l * N + j) * N + i; const double x = i * h, y = j * h, z = l * h;
+        double wx = 0, wz = 0;
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          const double xc = M_PI + sgn * (d0 + a * cos(z)), yc = M_PI; double best = INFINITY;
+          for (int px = -1; px <= 1; px++) for (int py = -1; py <= 1; py++) {
+            const double dx = x - xc + px * TWO_PI, dy = y - yc + py * TWO_PI; const double q = dx * dx + dy * dy;
+            best = q < best ? q : best;
+          }
+          const double om = sgn * G / (M_PI * sig * sig) * exp(-best / (sig * sig));
+          wz += om; wx += -sgn * a * sin(z) * om;
+        }
+        ox[n] = wx; oy[n] = 0; oz[n] = wz;
+      }
     }
     for (int c = 0; c < 3; c++) spectral_forward(s->sp, s->om[c], s->NL[2 * c], s->NL[2 * c + 1]);
     project(s, s->NL, 1);
@@ -380,33 +432,12 @@ int or_setIC(or_solver *s, int kind, const double *prm) {
           for (int c = 0; c < 6; c++) s->S[c][idx] /= k2;
         } } }
     project(s, s->S, 1);
+    if (kind == IC_TUBESZ && prm[3] > 0) {   /* perturbation: rms velocity prm[3] per component, seed 1, k0 = 4 */
+      random_field(s, s->K, 1.0, 4.0, 1.5 * prm[3] * prm[3]);
+      for (int c = 0; c < 6; c++) for (long n = 0; n < s->NS; n++) s->S[c][n] += s->K[c][n];
+    }
   } else if (kind == IC_RANDOM) {
-    mb32 g = { (uint32_t)(int32_t)prm[0] };
-    const double k0 = prm[1], E0 = prm[2];
-    for (int c = 0; c < 6; c++) memset(s->S[c], 0, 8 * s->NS);
-    for (int l = 0; l < N; l++) { const double kz = s->kw[l];
-      for (int j = 0; j < N; j++) { const double ky = s->kw[j]; const long row = ((long)l * N + j) * NH;
-        for (int i = 0; i < NH; i++) {
-          const long idx = row + i; if (!s->mask[idx]) continue;
-          const double k2 = s->kx[i] * s->kx[i] + ky * ky + kz * kz; if (k2 == 0) continue;
-          if (i == 0 && (kz < 0 || (kz == 0 && ky < 0))) continue;
-          const double k = sqrt(k2), amp = sqrt(pow(k, 4) * exp(-2 * (k / k0) * (k / k0)) / (4 * M_PI * k2));
-          for (int c = 0; c < 3; c++) {
-            for (int part = 0; part < 2; part++) {
-              double a = mb32_next(&g); a = a > 1e-12 ? a : 1e-12;
-              const double b = mb32_next(&g);
-              s->S[2 * c + part][idx] = amp * (sqrt(-2 * log(a)) * cos(TWO_PI * b));
-            }
-          }
-          if (i == 0) {
-            const int jm = (N - j) % N, lm = (N - l) % N; const long idm = ((long)lm * N + jm) * NH;
-            for (int c = 0; c < 3; c++) { s->S[2 * c][idm] = s->S[2 * c][idx]; s->S[2 * c + 1][idm] = -s->S[2 * c + 1][idx]; }
-          }
-        } } }
-    project(s, s->S, 1);
-    double E, Z, P; norms(s, s->S, &E, &Z, &P);
-    const double sc = sqrt(E0 / E);
-    for (int c = 0; c < 6; c++) for (long n = 0; n < s->NS; n++) s->S[c][n] *= sc;
+    random_field(s, s->S, prm[0], prm[1], prm[2]);
   } else return -1;
   s->st.t = 0; s->st.step = 0;
   finishIC(s);

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-IC_KINDS = {"tgv": 0, "tgv2d": 1, "abc": 2, "tubes": 3, "random": 4}
+IC_KINDS = {"tgv": 0, "tgv2d": 1, "abc": 2, "tubes": 3, "random": 4, "tubesz": 5}
 DIAG_KEYS = ["t", "step", "E", "Z", "P", "eps", "omMax", "uMax", "Pspec", "Pphys", "Tnl", "skew", "divMax", "eta",
              "kmaxEta", "Rel", "lambda", "skewIso"]
 REC_KEYS = ["t", "E", "Z", "eps", "omMax", "uMax", "dt", "ebal", "zbal", "Pspec", "Pal"]
@@ -77,6 +77,8 @@ def ic_params(kind, prm=None):
         return [g("A", 1.0), g("B", 1.0), g("C", 1.0), 0.0]
     if kind == "tubes":
         return [g("amp", 8.0), g("sigma", 0.4), g("sep", 0.7), g("pert", 0.2)]
+    if kind == "tubesz":   # the memory-heavy benchmark's tube pair along z (no reference counterpart)
+        return [g("Gamma", 1.0), g("sigma", 0.3), g("sep", 0.8), g("pert", 1e-4)]
     if kind == "random":
         seed = int(prm["seed"]) if prm.get("seed") else 1
         seed = ((seed + 2 ** 31) % 2 ** 32) - 2 ** 31

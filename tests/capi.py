@@ -11,7 +11,7 @@ PKG = os.path.join(ROOT, "build-me-anything.github.io_amd")
 LIB = os.path.join(PKG, "lib", "libnslab_hip.so")
 HEADER = os.path.join(ROOT, "include", "nslab_hip.h")
 
-IC = {"tgv": 0, "tgv2d": 1, "abc": 2, "tubes": 3, "random": 4}
+IC = {"tgv": 0, "tgv2d": 1, "abc": 2, "tubes": 3, "random": 4, "tubesz": 5}
 DIAG_KEYS = ["t", "step", "E", "Z", "P", "eps", "omMax", "uMax", "Pspec", "Pphys", "Tnl", "skew", "divMax", "eta",
              "kmaxEta", "Rel", "lambda", "skewIso"]
 LAST_KEYS = ["E", "Z", "P", "eps", "Tnl", "Pspec", "omMax", "uMax"]
@@ -128,6 +128,8 @@ def ic_params(kind, prm=None):
         return [g("A", 1.0), g("B", 1.0), g("C", 1.0), 0.0]
     if kind == "tubes":
         return [g("amp", 8.0), g("sigma", 0.4), g("sep", 0.7), g("pert", 0.2)]
+    if kind == "tubesz":   # the memory-heavy benchmark's tube pair along z (no reference counterpart)
+        return [g("Gamma", 1.0), g("sigma", 0.3), g("sep", 0.8), g("pert", 1e-4)]
     if kind == "random":
         seed = int(prm["seed"]) if prm.get("seed") else 1
         seed = ((seed + 2 ** 31) % 2 ** 32) - 2 ** 31

@@ -101,7 +101,7 @@ def test_tubes_ic_matches_reference():
 
 
 def test_hip_matches_oracle_adaptive_cfl():
-    for (N, ic, Re) in ((16, "abc", 10), (48, "tgv", 1600), (36, "random", 800)):
+    for (N, ic, Re) in ((16, "abc", 10), (48, "tgv", 1600), (36, "random", 800), (32, "tubesz", 2000)):
         s = capi.Solver(N=N, Re=Re, ic=ic, cfl=0.4)
         o = O.OracleSolver(N=N, Re=Re, ic=ic, cfl=0.4)
         assert s.run(12) == 12 and o.run(12) == 12

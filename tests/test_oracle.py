@@ -113,3 +113,57 @@ def test_planted_mistake_is_caught():
     bad = O.OracleSolver(N=16, nu=-0.1, ic="abc", dt=0.05)
     bad.run(20)
     assert rel_inf(bad.state(), good) > 1e-3
+
+
+def _curl_physical(S, N):
+    """omega = curl u on the grid from a state in the reference layout (f = sum u_k e^{ik.x})."""
+    NH = N // 2 + 1
+    U = [(S[2 * c] + 1j * S[2 * c + 1]).reshape(N, N, NH) for c in range(3)]
+    kw = np.fft.fftfreq(N, 1.0 / N)
+    kz, ky, kx = kw[:, None, None], kw[None, :, None], np.arange(NH)[None, None, :]
+    om = [1j * (ky * U[2] - kz * U[1]), 1j * (kz * U[0] - kx * U[2]), 1j * (kx * U[1] - ky * U[0])]
+    return [np.fft.irfftn(w, s=(N, N, N), axes=(0, 1, 2)) * N ** 3 for w in om]
+
+
+def test_tube_pair_along_z_is_the_stated_vorticity():
+    """The memory-heavy benchmark's initial field (no reference counterpart), pinned by what the construction fixes:
+    curl of the recovered velocity equals the analytic tube vorticity (Gaussian cores +-Gamma/(pi sigma^2), centres
+    (pi +- (d + 0.1 cos z), pi), omega along the centreline tangent) up to the 2/3 truncation of the Gaussian, the
+    circulation of each tube is -+Gamma, the velocity is solenoidal, and the perturbation carries energy
+    3/2 pert^2 (rms velocity pert per component)."""
+    N, G, sig, d0 = 64, 1.0, 0.3, 0.8
+    o = O.OracleSolver(N=N, Re=2000, ic="tubesz", icParams={"pert": 0}, cfl=0.5)
+    S = o.state()
+    wx, wy, wz = _curl_physical(S, N)
+    h = 2 * np.pi / N
+    z, y, x = np.meshgrid(np.arange(N) * h, np.arange(N) * h, np.arange(N) * h, indexing="ij")
+    ex = np.zeros_like(x); ez = np.zeros_like(x)
+    for sgn in (-1, 1):
+        xc = np.pi + sgn * (d0 + 0.1 * np.cos(z))
+        best = np.full_like(x, np.inf)
+        for px in (-1, 0, 1):
+            for py in (-1, 0, 1):
+                best = np.minimum(best, (x - xc + px * 2 * np.pi) ** 2 + (y - np.pi + py * 2 * np.pi) ** 2)
+        om = sgn * G / (np.pi * sig ** 2) * np.exp(-best / sig ** 2)
+        ez += om
+        ex += -sgn * 0.1 * np.sin(z) * om
+    peak = np.max(np.abs(ez))
+    assert np.max(np.abs(wz - ez)) < 2e-3 * peak
+    assert np.max(np.abs(wx - ex)) < 2e-3 * peak
+    assert np.max(np.abs(wy)) < 2e-3 * peak
+    # circulation of the tube at x < pi (sgn = -1) through the plane z = 0
+    circ = np.sum(wz[0][:, : N // 2]) * h * h
+    assert abs(circ + G) < 1e-3
+    # solenoidal: k . u = 0 mode by mode
+    NH = N // 2 + 1
+    kw = np.fft.fftfreq(N, 1.0 / N)
+    div = (np.arange(NH)[None, None, :] * (S[0] + 1j * S[1]).reshape(N, N, NH)
+           + kw[None, :, None] * (S[2] + 1j * S[3]).reshape(N, N, NH)
+           + kw[:, None, None] * (S[4] + 1j * S[5]).reshape(N, N, NH))
+    assert np.max(np.abs(div)) < 1e-14 * np.max(np.abs(S))
+    # the perturbation: the field with pert = 1e-4 minus the one without has E = 3/2 pert^2
+    p = O.OracleSolver(N=N, Re=2000, ic="tubesz", icParams={"pert": 1e-4}, cfl=0.5)
+    D = p.state() - S
+    NHw = np.where((np.arange(NH) == 0) | (np.arange(NH) == N // 2), 1.0, 2.0)
+    Ep = 0.5 * np.sum(NHw[None, None, :] * (D.reshape(6, N, N, NH) ** 2).sum(axis=0))
+    assert abs(Ep - 1.5e-8) < 1e-13
