@@ -180,7 +180,10 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, const double2 *
 #pragma unroll
       for (int j = 0; j < R; j++) {
         const int n = bb + j * NB;
-        v[k][j] = FIRST ? load(n, t) : lds[LY::at(n, t)];
+        if constexpr (!FIRST) v[k][j] = lds[LY::at(n, t)];
+        // a load functor may also take the register slot (k, j): inputs can come from registers
+        else if constexpr (std::is_invocable<LD, int, int, int, int>::value) v[k][j] = load(n, t, k, j);
+        else v[k][j] = load(n, t);
       }
     }
   }

@@ -19,6 +19,10 @@ static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
 #ifndef NSL_XNL_ROWS
 #define NSL_XNL_ROWS 1
 #endif
+// x pass with one wave per row at N = 512 (product and forward first stage in registers)
+#ifndef NSL_XNL_WAVE
+#define NSL_XNL_WAVE 1
+#endif
 static constexpr int fast_LP(int N) {
   return 6 * N * 8 <= NSL_LP_ELEMS ? 8 : 6 * N * 4 <= NSL_LP_ELEMS ? 4 : 6 * N * 2 <= NSL_LP_ELEMS ? 2 : 1;
 }
@@ -99,7 +103,11 @@ template <int N>
 static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
   int nblk;
 #if NSL_XNL_ROWS
-  if constexpr (N >= 512) {
+  if constexpr (N == 512 && NSL_XNL_WAVE) {
+    const int lds = fast::LineMajor<N, 1>::SIZE * (int)sizeof(double2);
+    nblk = N * N;
+    hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(N / 8), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+  } else if constexpr (N >= 512) {
     constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
     const int lds = N * 3 * RP * (int)sizeof(double2);
     allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);

@@ -443,6 +443,84 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
   }
 }
 
+// P3 with one wave per x row (N = 512: 8 x 8 x 8, 64 butterflies per stage, one per lane). Every lane owns
+// butterfly bb = lane of all three lines (u_x + i u_y), (u_z + i om_x), (om_y + i om_z), so the product u x omega is
+// formed in registers from the inverse's last stage, and the forward's first stage starts from those registers:
+// LDS carries only the two exchanges inside each line's transform, one line at a time (a 9 KB buffer), and the
+// half-spectra are read straight from HBM into the first stage (Hermitian mirror of modes N-k in place).
+#ifndef NSL_XNL_WAVE_MINW
+#define NSL_XNL_WAVE_MINW 1
+#endif
+template <int N>
+__global__ __launch_bounds__(N / 8, NSL_XNL_WAVE_MINW) void kf_xnl_wave(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+                                                     double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  static_assert(N == 512, "one wave per row needs 64 first-stage and last-stage butterflies of radix 8");
+  constexpr int NT = N / 8, R = 8;
+  X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
+  if (ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int hx = g.hx, HP = g.hp;
+  constexpr size_t cs = (size_t)N * N * HP;
+  using LY = LineMajor<N, 1>;
+  const int row = blockIdx.x;
+  const size_t base = (size_t)row * HP;
+  const int lane = threadIdx.x;
+  double2 U[3][R];
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const double2 *Fa = X2 + 2 * t * cs + base, *Fb = X2 + (2 * t + 1) * cs + base;
+    auto ld = [&](int n, int) -> double2 {
+      if (n < hx) {
+        const double2 A = Fa[n], B = Fb[n];
+        return make_double2(A.x - B.y, A.y + B.x);
+      }
+      if (n > N - hx) {
+        const double2 A = Fa[N - n], B = Fb[N - n];
+        return make_double2(A.x + B.y, B.x - A.y);
+      }
+      return make_double2(0.0, 0.0);
+    };
+    auto keep = [&](int, int, double2 v, int, int j) { U[t][j] = v; };
+    fft<N, 1, NT, 1, false, LY>(lds, tw, ld, keep);
+  }
+  // u x omega at x = lane + 64 j, the inverse's outputs of this lane
+  double om2 = 0.0, u2 = 0.0;
+  double2 P[2][R];
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    const double a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
+    om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c);
+    P[0][j] = make_double2(b * rr - c * q, c * pp - a * rr);
+    P[1][j] = make_double2(a * q - b * pp, 0.0);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    auto ld2 = [&](int, int, int, int j) -> double2 { return P[t][j]; };
+    auto st2 = [&](int n, int, double2 v) {   // only modes k < hx and their mirrors N-k are unpacked
+      if (n < hx || n > N - hx) lds[LY::at(n, 0)] = v;
+    };
+    fft<N, 1, NT, -1, false, LY>(lds, tw, ld2, st2);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < (hx + NT - 1) / NT; i++) {
+      const int k = lane + i * NT;
+      if (k >= hx) continue;
+      const double2 Zk = lds[LY::at(k, 0)], Zc = lds[LY::at((N - k) % N, 0)];
+      if (t == 0) {   // n_x + i n_y
+        const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+        X2[base + k] = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+        X2[cs + base + k] = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+      } else {        // n_z (imaginary part zero)
+        X2[2 * cs + base + k] = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+      }
+    }
+    __syncthreads();   // the next line's first stage overwrites LDS
+  }
+  om2 = wave_max(om2); u2 = wave_max(u2);
+  if (lane == 0) { pmax[2 * (size_t)row] = om2; pmax[2 * (size_t)row + 1] = u2; }
+}
+
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
 // workgroups, one atomic pair each (the max is exact, so the order does not matter).
 __global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl,
