@@ -1,0 +1,58 @@
+// The long-run harness (build-me-anything.github.io_amd/js/run_long.js) on a GPU box: on-disk formats, snapshot
+// cadence, checkpoint/resume bit-identity, the --compare path and the parity gate. Exit 0 and "ALL PASS" on success.
+'use strict';
+const fs = require('fs');
+const os = require('os');
+const path = require('path');
+const pkg = path.join(__dirname, '..', '..', 'build-me-anything.github.io_amd', 'js');
+const RL = require(path.join(pkg, 'run_long.js'));
+const { parityCheck } = require(path.join(pkg, 'parity_check.js'));
+
+let fails = 0;
+function check(name, ok, detail) {
+  if (!ok) fails++;
+  console.log(`${ok ? 'PASS' : 'FAIL'}  ${name}${detail != null ? '   ' + detail : ''}`);
+}
+const base = fs.mkdtempSync(path.join(os.tmpdir(), 'nsl-long-'));
+const args = (out, extra) => RL.parseArgs(['--N', '32', '--Re', '1600', '--tEnd', '1.2', '--snap', '0.5', '--ckpt', '0.4',
+  '--chunk', '8', '--out', out].concat(extra || []));
+const quiet = fn => { const l = console.log; console.log = () => {}; try { return fn(); } finally { console.log = l; } };
+
+const A = path.join(base, 'a'), B = path.join(base, 'b');
+quiet(() => RL.runLong(args(A)));
+const fa = JSON.parse(fs.readFileSync(path.join(A, 'final.json'), 'utf8'));
+check('final.json: case, precision, series, snapshots, health', fa.case.N === 32 && fa.precision === 'float64' && fa.final === true &&
+  ['t', 'E', 'Z', 'eps', 'omMax', 'uMax', 'dt', 'Pspec', 'Pal'].every(k => fa.series[k].length === fa.steps + 1) &&
+  ['ebal', 'zbal'].every(k => fa.series[k].length === fa.steps) &&
+  ['PASS', 'WARN', 'FAIL'].includes(fa.health.worst));
+check('snapshots at t = 0 and the first steps past 0.5, 1.0, then the end', fa.snapshots.length === 4 &&
+  fa.snapshots[0].t === 0 && fa.snapshots[1].t >= 0.5 - 1e-9 && fa.snapshots[2].t >= 1.0 - 1e-9 && fa.snapshots[3].t >= 1.2 - 1e-12,
+  fa.snapshots.map(o => o.t.toFixed(3)).join(' '));
+check('run ends at the first step reaching tEnd', fa.t >= 1.2 - 1e-12 && fa.t - fa.series.dt[fa.steps] < 1.2 - 1e-12, fa.t);
+const sl = fs.readdirSync(path.join(A, 'slices'));
+check('slices: 3 kinds x 4 planes per snapshot, N^2 float32 each', sl.length === 48 &&
+  sl.every(f => fs.statSync(path.join(A, 'slices', f)).size === 32 * 32 * 4), sl.length);
+const NSm = 32 * 32 * 17;
+check('checkpoint.bin holds the six state planes', fs.statSync(path.join(A, 'checkpoint.bin')).size === 6 * NSm * 8);
+check('run.log records snapshots and DONE', /DONE: ε_max/.test(fs.readFileSync(path.join(A, 'run.log'), 'utf8')));
+
+// interrupted at t = 0.85 (checkpoints at 0.4, 0.8), then resumed to 1.2: identical to the uninterrupted run
+quiet(() => RL.runLong(RL.parseArgs(['--N', '32', '--Re', '1600', '--tEnd', '0.85', '--snap', '0.5', '--ckpt', '0.4',
+  '--chunk', '8', '--out', B])));
+quiet(() => RL.runLong(args(B)));
+const fb = JSON.parse(fs.readFileSync(path.join(B, 'final.json'), 'utf8'));
+check('resumed run log says so', /resumed from checkpoint at t=0\.8/.test(fs.readFileSync(path.join(B, 'run.log'), 'utf8')));
+check('resumed run equals the uninterrupted run bit for bit', fb.steps === fa.steps &&
+  ['t', 'E', 'Z', 'eps', 'omMax', 'ebal', 'zbal'].every(k => fb.series[k].every((v, i) => v === fa.series[k][i])) &&
+  fb.snapshots.length === fa.snapshots.length);
+const pc = parityCheck(fb, fa, () => {});
+check('parity gate passes the two runs', pc.verdict === 'PASS');
+const bad = JSON.parse(JSON.stringify(fb)); bad.series.E = bad.series.E.map((v, i) => v * (1 + 1e-5 * i / bad.series.E.length));
+check('parity gate fails a drifted run', parityCheck(bad, fa, () => {}).verdict === 'FAIL');
+// --compare replays the archived case and reports zero difference
+const cmp = quiet(() => RL.runLong(args(path.join(base, 'c'), ['--compare', path.join(A, 'final.json')])));
+check('--compare against the archive: zero difference', Object.values(cmp.compare).every(v => v === 0) && cmp.steps === fa.steps + 1,
+  JSON.stringify(cmp));
+fs.rmdirSync(base, { recursive: true });
+console.log(fails ? `${fails} FAILED` : 'ALL PASS');
+process.exit(fails ? 1 : 0);

@@ -46,3 +46,29 @@ def test_pools_and_request_loop_on_gpu():
     r = subprocess.run([node, script], capture_output=True, text=True, timeout=600)
     print(r.stdout)
     assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
+@pytest.mark.gpu
+def test_long_run_harness_on_gpu():
+    script = os.path.join(ROOT, "tests", "js", "validate_run_long.js")
+    r = subprocess.run([node, script], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
+def test_parity_gate_grades_series():
+    """No GPU needed: the gate of cloud/parity-check.py over synthetic final.json pairs."""
+    gate = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "parity_check.js")
+    js = ("const {parityCheck} = require(%r);"
+          "const mk = (f, dev) => ({case: {N: 64, Re: 1600, ic: 'tgv', cfl: 0.4}, precision: 'float64', device: dev,"
+          " series: Object.fromEntries(['t','E','Z','eps','omMax','Pspec'].map(k => [k, Array.from({length: 50},"
+          " (_, i) => (k === 't' ? 0.01 * i : 1 + 0.01 * i) * (k === 't' ? 1 : f(i)))]))});"
+          "const q = () => {}; const a = mk(() => 1, 'x');"
+          "const r1 = parityCheck(mk(() => 1, 'y'), a, q).verdict, r2 = parityCheck(mk(i => 1 + 1e-8 * i / 50, 'y'), a, q).verdict,"
+          " r3 = parityCheck(mk(i => 1 + 1e-4, 'y'), a, q).verdict;"
+          "const c = mk(() => 1, 'y'); c.case.N = 32; const r4 = parityCheck(c, a, q).verdict;"
+          "console.log([r1, r2, r3, r4].join(' '));") % gate
+    r = subprocess.run([node, "-e", js], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "PASS WARN FAIL FAIL", r.stdout + r.stderr
