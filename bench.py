@@ -3,10 +3,12 @@
 
 Workload (the headline configuration): Taylor-Green vortex on [0, 2pi]^3 at 256^3, nu = 1/1600, fixed dt = 0.004,
 FP64, 2/3 dealiasing, rotational form, classical RK4. A "step" is one RK4 step (4 right-hand sides). The state is
-resident in HBM when the timed region starts; the timed region is K steps between two device synchronisations.
+resident in HBM when the timed region starts; the timed region is K steps between two device synchronisations, with
+nothing but the solver's kernels in the stream. The same K steps then run again in a second timed region with a HIP
+event pair around every launch, for the per-kernel averages (the events add gaps between kernels, not kernel time).
 
 Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
-its mean launch time, from HIP events recorded on the solver's stream inside the timed region), the whole-step
+its mean launch time, from HIP events recorded on the solver's stream inside the second timed region), the whole-step
 roofline fraction of the truncated-storage traffic model, a 512^3 line, the CPU baseline (the oracle, a
 single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and two lines from
 the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step) and the
@@ -68,18 +70,25 @@ def measure(N, steps, warmup, dt, label):
     s = capi.Solver(N=N, Re=1600, ic="tgv", dt=dt)
     s.run(warmup)
     torch.cuda.synchronize()
-    L = capi.lib()
-    capi.check(L.nsl_profile(s.h, 1))
-    torch.cuda.synchronize()
+    # timed region 1: the throughput, no instrumentation in the stream
     t0 = time.perf_counter()
     done = s.run(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # timed region 2: the same number of steps with a HIP event pair around every launch (nsl_profile) for the
+    # per-kernel averages; the events add gaps between kernels, not kernel time
+    L = capi.lib()
+    capi.check(L.nsl_profile(s.h, 1))
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    done_p = s.run(steps)
+    torch.cuda.synchronize()
+    tp1 = time.perf_counter()
     ms = (ctypes.c_double * len(CLASSES))()
     cnt = (ctypes.c_int64 * len(CLASSES))()
     capi.check(L.nsl_profile_read(s.h, ms, cnt))
     capi.check(L.nsl_profile(s.h, 0))
-    assert done == steps, (done, steps)
+    assert done == steps and done_p == steps, (done, done_p, steps)
     elapsed = t1 - t0
     per = alg_bytes_per_launch(N)
     kern = {}
@@ -93,7 +102,8 @@ def measure(N, steps, warmup, dt, label):
     s.close()
     del s
     return {"label": label, "N": N, "steps": steps, "elapsed_s": elapsed, "steps_per_s": steps / elapsed,
-            "ms_per_step": 1e3 * elapsed / steps, "kernels": kern, "dominant": dom, "E_end": E,
+            "ms_per_step": 1e3 * elapsed / steps, "ms_per_step_profiled": 1e3 * (tp1 - tp0) / steps,
+            "kernels": kern, "dominant": dom, "E_end": E,
             "step_roofline_frac": b_alg_step(N) * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
 
 
@@ -234,6 +244,7 @@ def main():
                      "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": k["alg_bytes"], "avg_launch_us": k["avg_us"]},
         "step_roofline_frac": main_r["step_roofline_frac"],
+        "ms_per_step_profiled": main_r["ms_per_step_profiled"],
         "b_alg_per_step_GB": b_alg_step(a.N) / 1e9,
         "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1), "launches": v["launches"]}
                     for n, v in main_r["kernels"].items()},
