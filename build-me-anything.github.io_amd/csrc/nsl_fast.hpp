@@ -246,6 +246,11 @@ template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleave
 __device__ __forceinline__ void fft(double2 *lds, const double2 *tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, typename Sched<N>::L{});
 }
+// The same with an explicit radix schedule SCH (an RList whose product is N).
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class LD, class ST>
+__device__ __forceinline__ void fft_sched(double2 *lds, const double2 *tw, LD &load, ST &store) {
+  run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, SCH{});
+}
 
 // ------------------------------------------------------------------------------------------------ split form
 // The first stage's inputs held in registers, so a persistent workgroup can issue the loads of its next tile before

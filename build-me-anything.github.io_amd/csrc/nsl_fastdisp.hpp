@@ -19,9 +19,12 @@ static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
 #ifndef NSL_XNL_ROWS
 #define NSL_XNL_ROWS 1
 #endif
-// x pass with one wave per row at N = 512 (product and forward first stage in registers)
+// x pass with one wave per row at N = 512, per two rows at N = 256 (product and forward first stage in registers)
 #ifndef NSL_XNL_WAVE
 #define NSL_XNL_WAVE 1
+#endif
+#ifndef NSL_XNL_WAVE256
+#define NSL_XNL_WAVE256 0   // 256^3: the row-pair form (16 x 16, one LDS exchange) measured faster than 8 x 4 x 8
 #endif
 static constexpr int fast_LP(int N) {
   return 6 * N * 8 <= NSL_LP_ELEMS ? 8 : 6 * N * 4 <= NSL_LP_ELEMS ? 4 : 6 * N * 2 <= NSL_LP_ELEMS ? 2 : 1;
@@ -103,10 +106,11 @@ template <int N>
 static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
   int nblk;
 #if NSL_XNL_ROWS
-  if constexpr (N == 512 && NSL_XNL_WAVE) {
-    const int lds = fast::LineMajor<N, 1>::SIZE * (int)sizeof(double2);
-    nblk = N * N;
-    hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(N / 8), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+  if constexpr ((N == 512 || (N == 256 && NSL_XNL_WAVE256)) && NSL_XNL_WAVE) {
+    constexpr int RPW = 64 / (N / 8);
+    const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
+    nblk = N * N / RPW;
+    hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
   } else if constexpr (N >= 512) {
     constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
     const int lds = N * 3 * RP * (int)sizeof(double2);

@@ -443,34 +443,37 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
   }
 }
 
-// P3 with one wave per x row (N = 512: 8 x 8 x 8, 64 butterflies per stage, one per lane). Every lane owns
-// butterfly bb = lane of all three lines (u_x + i u_y), (u_z + i om_x), (om_y + i om_z), so the product u x omega is
-// formed in registers from the inverse's last stage, and the forward's first stage starts from those registers:
-// LDS carries only the two exchanges inside each line's transform, one line at a time (a 9 KB buffer), and the
-// half-spectra are read straight from HBM into the first stage (Hermitian mirror of modes N-k in place).
-#ifndef NSL_XNL_WAVE_MINW
-#define NSL_XNL_WAVE_MINW 1
-#endif
+// P3 with one wave per x row (or per two rows at N = 256). The radix schedule starts and ends with radix 8, so a
+// line has N/8 first- and last-stage butterflies: 64 at N = 512 (one row per wave, 8 x 8 x 8) and 32 at N = 256 (two
+// rows per wave, 8 x 4 x 8). Every lane owns butterfly bb of all three packed lines (u_x + i u_y), (u_z + i om_x),
+// (om_y + i om_z) of its row, so the product u x omega is formed in registers from the inverse's last stage (outputs
+// x = bb + (N/8) j) and the forward's first stage starts from those registers (inputs at the same x): LDS carries
+// only the exchanges inside each line's transform, one line of each row at a time, and the half-spectra are read
+// straight from HBM into the first stage (Hermitian mirror of modes N-k in place).
+template <int N> struct XnlWave;
+template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
+template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
 template <int N>
-__global__ __launch_bounds__(N / 8, NSL_XNL_WAVE_MINW) void kf_xnl_wave(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
-                                                     double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
-  static_assert(N == 512, "one wave per row needs 64 first-stage and last-stage butterflies of radix 8");
-  constexpr int NT = N / 8, R = 8;
+__global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+                                                  double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  constexpr int R = 8, NB = N / R, RPW = 64 / NB;   // butterflies per line and stage ends, rows per wave
+  static_assert(NB * RPW == 64, "first and last stage must cover a wave");
+  using SCH = typename XnlWave<N>::L;
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
   constexpr size_t cs = (size_t)N * N * HP;
-  using LY = LineMajor<N, 1>;
-  const int row = blockIdx.x;
-  const size_t base = (size_t)row * HP;
+  using LY = LineMajor<N, RPW>;
   const int lane = threadIdx.x;
+  const int row0 = blockIdx.x * RPW;
   double2 U[3][R];
 #pragma unroll
-  for (int t = 0; t < 3; t++) {
-    const double2 *Fa = X2 + 2 * t * cs + base, *Fb = X2 + (2 * t + 1) * cs + base;
-    auto ld = [&](int n, int) -> double2 {
+  for (int l = 0; l < 3; l++) {
+    auto ld = [&](int n, int t) -> double2 {   // line l of row row0 + t
+      const size_t base = (size_t)(row0 + t) * HP;
+      const double2 *Fa = X2 + 2 * l * cs + base, *Fb = X2 + (2 * l + 1) * cs + base;
       if (n < hx) {
         const double2 A = Fa[n], B = Fb[n];
         return make_double2(A.x - B.y, A.y + B.x);
@@ -481,10 +484,10 @@ __global__ __launch_bounds__(N / 8, NSL_XNL_WAVE_MINW) void kf_xnl_wave(double2 
       }
       return make_double2(0.0, 0.0);
     };
-    auto keep = [&](int, int, double2 v, int, int j) { U[t][j] = v; };
-    fft<N, 1, NT, 1, false, LY>(lds, tw, ld, keep);
+    auto keep = [&](int, int, double2 v, int, int j) { U[l][j] = v; };
+    fft_sched<N, RPW, 64, 1, false, LY, SCH>(lds, tw, ld, keep);
   }
-  // u x omega at x = lane + 64 j, the inverse's outputs of this lane
+  // u x omega at x = bb + NB j of row row0 + lane / NB: the inverse's outputs of this lane
   double om2 = 0.0, u2 = 0.0;
   double2 P[2][R];
 #pragma unroll
@@ -495,19 +498,21 @@ __global__ __launch_bounds__(N / 8, NSL_XNL_WAVE_MINW) void kf_xnl_wave(double2 
     P[1][j] = make_double2(a * q - b * pp, 0.0);
   }
 #pragma unroll
-  for (int t = 0; t < 2; t++) {
-    auto ld2 = [&](int, int, int, int j) -> double2 { return P[t][j]; };
-    auto st2 = [&](int n, int, double2 v) {   // only modes k < hx and their mirrors N-k are unpacked
-      if (n < hx || n > N - hx) lds[LY::at(n, 0)] = v;
+  for (int l = 0; l < 2; l++) {
+    auto ld2 = [&](int, int, int, int j) -> double2 { return P[l][j]; };
+    auto st2 = [&](int n, int t, double2 v) {   // only modes k < hx and their mirrors N-k are unpacked
+      if (n < hx || n > N - hx) lds[LY::at(n, t)] = v;
     };
-    fft<N, 1, NT, -1, false, LY>(lds, tw, ld2, st2);
+    fft_sched<N, RPW, 64, -1, false, LY, SCH>(lds, tw, ld2, st2);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < (hx + NT - 1) / NT; i++) {
-      const int k = lane + i * NT;
-      if (k >= hx) continue;
-      const double2 Zk = lds[LY::at(k, 0)], Zc = lds[LY::at((N - k) % N, 0)];
-      if (t == 0) {   // n_x + i n_y
+    for (int i = 0; i < (RPW * hx + 63) / 64; i++) {
+      const int e = lane + i * 64;
+      if (e >= RPW * hx) continue;
+      const int t = e / hx, k = e - t * hx;
+      const size_t base = (size_t)(row0 + t) * HP;
+      const double2 Zk = lds[LY::at(k, t)], Zc = lds[LY::at((N - k) % N, t)];
+      if (l == 0) {   // n_x + i n_y
         const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
         X2[base + k] = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
         X2[cs + base + k] = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
@@ -518,7 +523,7 @@ __global__ __launch_bounds__(N / 8, NSL_XNL_WAVE_MINW) void kf_xnl_wave(double2 
     __syncthreads();   // the next line's first stage overwrites LDS
   }
   om2 = wave_max(om2); u2 = wave_max(u2);
-  if (lane == 0) { pmax[2 * (size_t)row] = om2; pmax[2 * (size_t)row + 1] = u2; }
+  if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
 }
 
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
