@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #define NSL_NT 256          // threads per workgroup (4 wave64)
 #define NSL_MAXST 16        // max FFT stages (radix >= 2 -> N <= 2^16)
@@ -143,9 +144,12 @@ __device__ void lds_fft(double2 *__restrict__ buf, int T, const DPlan &P, double
 // A pool holds several solves of one N, slot z's buffers sb bytes after slot z-1's. A pass launched with
 // gridDim.z = slots runs every slot at once, each workgroup on the buffers of slot blockIdx.z; sb = 0 with one slot
 // is the single-solve case.
+// (byte arithmetic on the pointer itself, not through an integer, so the compiler still sees a global-memory
+// pointer and emits global rather than flat loads and stores)
 template <class T>
 __device__ __forceinline__ T *at_slot(T *p, size_t sb) {
-  return p == nullptr ? p : reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) + (size_t)blockIdx.z * sb);
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return p == nullptr ? p : reinterpret_cast<T *>(reinterpret_cast<B *>(p) + (size_t)blockIdx.z * sb);
 }
 
 // ------------------------------------------------------------------------------------------------ reductions

@@ -164,11 +164,61 @@ template <int N, int T> struct LineMajor {
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { bb = idx % NB; t = idx / NB; }
 };
 
+// Twiddle table values of one thread for every stage of a radix schedule, read once. A kernel that runs several
+// transforms in the same layout (the x pass: five per row) reads them before the first transform and keeps them in
+// registers, so no stage waits on a table read. Slot j of stage S holds tw[j * p * S] for the j that twiddles<R>
+// reads from the table; the other powers are formed from these exactly as twiddles<R> forms them.
+__host__ __device__ constexpr bool tw_from_table(int j) { return !NSL_TW_PRODUCTS || j < 4 || (j & 3) == 0; }
+template <int N, int T, int NT, class LY, int S, class L> struct TwPre {
+  __device__ __forceinline__ void load(const double2 *__restrict__) {}
+};
+template <int N, int T, int NT, class LY, int S, int R0, int... Rs> struct TwPre<N, T, NT, LY, S, RList<R0, Rs...>> {
+  static constexpr int NB = N / R0, TOT = NB * T, K = (TOT + NT - 1) / NT;
+  static constexpr bool HAS = S * R0 != N;
+  double2 w[HAS ? K : 1][R0];
+  TwPre<N, T, NT, LY, S * R0, RList<Rs...>> next;
+  __device__ __forceinline__ void load(const double2 *__restrict__ tw) {
+    if constexpr (HAS) {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int idx = threadIdx.x + k * NT;
+        int t, bb;
+        LY::template split<NB>(idx, t, bb);
+        const int base = (bb / S) * S;
+#pragma unroll
+        for (int j = 1; j < R0; j++)
+          if (tw_from_table(j)) w[k][j] = (TOT % NT == 0 || idx < TOT) ? tw[j * base] : make_double2(1.0, 0.0);
+      }
+    }
+    next.load(tw);
+  }
+  template <int SS> __device__ __forceinline__ auto &at() {
+    if constexpr (SS == S) return *this;
+    else return next.template at<SS>();
+  }
+};
+template <int N, int T, int NT, class LY, class SCH> using TwCache = TwPre<N, T, NT, LY, 1, SCH>;
+
+// The twiddles of slot k of a stage: from the table (TWS a pointer) or from a TwCache.
+template <int R, int S, class TWS>
+__device__ __forceinline__ void stage_twiddles(TWS &tws, int k, int base, double2 (&w)[R]) {
+  if constexpr (!std::is_class<typename std::remove_cv<TWS>::type>::value) {   // the table itself
+    twiddles<R>(tws, base, w);
+  } else {
+    auto &st = tws.template at<S>();
+#pragma unroll
+    for (int j = 1; j < R; j++) {
+      if (tw_from_table(j)) w[j] = st.w[k][j];
+      else w[j] = cmul(w[j & ~3], w[j & 3]);
+    }
+  }
+}
+
 // One stage. S = product of the radices before it. Input either from `load(n, t)` (FIRST) or LDS; output to
 // `store(n, t, v)` (LAST) or LDS, at the layout LY's addresses.
-template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, class LY, class LD,
-          class ST>
-__device__ __forceinline__ void stage(double2 *__restrict__ lds, const double2 *__restrict__ tw, LD &load, ST &store) {
+template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, class LY, class TWS,
+          class LD, class ST>
+__device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &load, ST &store) {
   constexpr int NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
   double2 v[K][R];
 #pragma unroll
@@ -198,7 +248,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, const double2 *
       Dft<R, SIGN>::run(v[k]);
       if (S * R != N) {
         double2 w[R];
-        twiddles<R>(tw, p * S, w);
+        stage_twiddles<R, S>(tw, k, p * S, w);
 #pragma unroll
         for (int j = 1; j < R; j++) v[k][j] = mul_tw<SIGN>(v[k][j], w[j]);
       }
@@ -232,9 +282,9 @@ template <int N, int T, int NT> struct OutRegs {
   static __device__ __forceinline__ int elem(int k, int j) { return ((int)threadIdx.x + k * NT) / T + j * S; }
 };
 
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, class LD, class ST, int R0,
-          int... Rs>
-__device__ __forceinline__ void run_stages(double2 *lds, const double2 *tw, LD &load, ST &store, RList<R0, Rs...>) {
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, class TWS, class LD, class ST,
+          int R0, int... Rs>
+__device__ __forceinline__ void run_stages(double2 *lds, TWS &tw, LD &load, ST &store, RList<R0, Rs...>) {
   constexpr bool LAST = sizeof...(Rs) == 0;
   stage<N, T, NT, R0, S, FIRST, LAST, SIGN, SYNC_FIRST, LY>(lds, tw, load, store);
   if constexpr (!LAST) run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, S * R0, false>(lds, tw, load, store, RList<Rs...>{});
@@ -242,13 +292,14 @@ __device__ __forceinline__ void run_stages(double2 *lds, const double2 *tw, LD &
 
 // Full transform of T lines of length N. tw: the N roots of unity (cos, sin)(2 pi k / N). SYNC_FIRST: the first
 // stage's loads read LDS that the stage then overwrites (in-place use), so a barrier separates them.
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleaved<N, T>, class LD, class ST>
-__device__ __forceinline__ void fft(double2 *lds, const double2 *tw, LD &load, ST &store) {
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleaved<N, T>, class TWS, class LD, class ST>
+__device__ __forceinline__ void fft(double2 *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, typename Sched<N>::L{});
 }
-// The same with an explicit radix schedule SCH (an RList whose product is N).
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class LD, class ST>
-__device__ __forceinline__ void fft_sched(double2 *lds, const double2 *tw, LD &load, ST &store) {
+// The same with an explicit radix schedule SCH (an RList whose product is N); tw is the table or a
+// TwCache<N, T, NT, LY, SCH>.
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
+__device__ __forceinline__ void fft_sched(double2 *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, SCH{});
 }
 

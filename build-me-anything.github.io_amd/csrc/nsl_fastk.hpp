@@ -468,24 +468,27 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
   using LY = LineMajor<N, RPW>;
   const int lane = threadIdx.x;
   const int row0 = blockIdx.x * RPW;
+  // the twiddles of this lane's butterflies are the same in all five transforms: read once
+  TwCache<N, RPW, 64, LY, SCH> twc;
+  twc.load(tw);
   double2 U[3][R];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
-    auto ld = [&](int n, int t) -> double2 {   // line l of row row0 + t
+    // line l of row row0 + t: mode n < hx from the half-spectrum, n > N - hx from its mirror, zero between. Every
+    // load is unconditional (the zero band reads index 0 and discards it), so a line's loads issue back to back and
+    // are waited on once; slots j whose range j*NB .. j*NB+NB-1 lies in the zero band load nothing.
+    auto ld = [&](int n, int t, int, int j) -> double2 {
+      if (j * NB >= hx && j * NB + NB - 1 <= N - hx) return make_double2(0.0, 0.0);
       const size_t base = (size_t)(row0 + t) * HP;
       const double2 *Fa = X2 + 2 * l * cs + base, *Fb = X2 + (2 * l + 1) * cs + base;
-      if (n < hx) {
-        const double2 A = Fa[n], B = Fb[n];
-        return make_double2(A.x - B.y, A.y + B.x);
-      }
-      if (n > N - hx) {
-        const double2 A = Fa[N - n], B = Fb[N - n];
-        return make_double2(A.x + B.y, B.x - A.y);
-      }
-      return make_double2(0.0, 0.0);
+      const bool lo = n < hx, hi = n > N - hx;
+      const int m = lo ? n : (hi ? N - n : 0);
+      const double2 A = Fa[m], B = Fb[m];
+      const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
+      return lo || hi ? d : make_double2(0.0, 0.0);
     };
     auto keep = [&](int, int, double2 v, int, int j) { U[l][j] = v; };
-    fft_sched<N, RPW, 64, 1, false, LY, SCH>(lds, tw, ld, keep);
+    fft_sched<N, RPW, 64, 1, false, LY, SCH>(lds, twc, ld, keep);
   }
   // u x omega at x = bb + NB j of row row0 + lane / NB: the inverse's outputs of this lane
   double om2 = 0.0, u2 = 0.0;
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
     auto st2 = [&](int n, int t, double2 v) {   // only modes k < hx and their mirrors N-k are unpacked
       if (n < hx || n > N - hx) lds[LY::at(n, t)] = v;
     };
-    fft_sched<N, RPW, 64, -1, false, LY, SCH>(lds, tw, ld2, st2);
+    fft_sched<N, RPW, 64, -1, false, LY, SCH>(lds, twc, ld2, st2);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < (RPW * hx + 63) / 64; i++) {
