@@ -193,22 +193,23 @@ __device__ __forceinline__ void block_max2_store(double a, double b, double *dst
 
 // Fixed-order tree sum of NV values per thread across the workgroup; thread 0 gets the totals in out[].
 // The order depends only on the launch shape, so results are reproducible run to run and batch to batch.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch /* NSL_NT*NV */, double *out) {
+template <int NV, int NT = NSL_NT>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch /* NT*NV */, double *out) {
+  static_assert((NT & (NT - 1)) == 0, "power-of-two workgroup");
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < NV; i++) scratch[i * NSL_NT + threadIdx.x] = v[i];
+  for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] = v[i];
   __syncthreads();
-  for (int w = NSL_NT / 2; w > 0; w >>= 1) {
+  for (int w = NT / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
 #pragma unroll
-      for (int i = 0; i < NV; i++) scratch[i * NSL_NT + threadIdx.x] += scratch[i * NSL_NT + threadIdx.x + w];
+      for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] += scratch[i * NT + threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) out[i] = scratch[i * NSL_NT];
+    for (int i = 0; i < NV; i++) out[i] = scratch[i * NT];
   }
 }
 

@@ -166,3 +166,41 @@ static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, i
 static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt)
 }
+
+// z-forward fused with the stage-`stage` update (kf_zfwd_upd) where it is compiled: N = 256 and 512.
+#ifndef NSL_ZU
+#define NSL_ZU 1
+#endif
+static bool fast_has_zu(const Xform &x) { return NSL_ZU && x.fast && (x.g.N == 256 || x.g.N == 512); }
+// Workgroups (and sum partials) of the fused pass.
+static int fast_zu_blocks(const Geo &g) { return ((g.hx + NSL_ZU_T - 1) / NSL_ZU_T) * g.by; }
+template <int N, int STAGE>
+static void fast_zu_launch(const Xform &x, const double2 *X4, const double2 *U, double2 *S, double2 *ACC, double2 *TS,
+                           double *partials, Ctl *ctl, const Batch &bt) {
+  using C = fast::ZuCfg<N>;
+  const int lds = N * C::T * (int)sizeof(double2);
+  const double inv = 1.0 / ((double)N * N * N);
+  allow_lds(fast::kf_zfwd_upd<N, STAGE>, lds);
+  hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE>), dim3((unsigned)fast_zu_blocks(x.g), 1, bt.nz), dim3(C::NT), lds, x.st,
+                     X4, U, S, ACC, TS, x.g, x.twN, inv, partials, ctl, bt.sb);
+}
+template <int N>
+static int fast_zu_n(const Xform &x, int stage, const double2 *X4, const double2 *U, double2 *S, double2 *ACC,
+                     double2 *TS, double *partials, Ctl *ctl, const Batch &bt) {
+  switch (stage) {
+    case 1: fast_zu_launch<N, 1>(x, X4, U, S, ACC, TS, partials, ctl, bt); break;
+    case 2: fast_zu_launch<N, 2>(x, X4, U, S, ACC, TS, partials, ctl, bt); break;
+    case 3: fast_zu_launch<N, 3>(x, X4, U, S, ACC, TS, partials, ctl, bt); break;
+    default: fast_zu_launch<N, 4>(x, X4, U, S, ACC, TS, partials, ctl, bt); break;
+  }
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+static int fast_dispatch_zu(const Xform &x, int stage, const double2 *X4, const double2 *U, double2 *S, double2 *ACC,
+                            double2 *TS, double *partials, Ctl *ctl, const Batch &bt) {
+  switch (x.g.N) {
+    case 256: return fast_zu_n<256>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 512: return fast_zu_n<512>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    default: return fail(NSL_ERR_STATE, "no fused z-forward for this N");
+  }
+}

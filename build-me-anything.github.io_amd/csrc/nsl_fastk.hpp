@@ -202,6 +202,98 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
 
+// P5 fused with the spectral update: one workgroup transforms all three components of its T lines (kx0..kx0+T-1
+// at row cy) along z, keeps the retained outputs in registers and applies update_mode<STAGE> to each of them, so the
+// nonlinear term never goes to HBM. The radix schedule ends in radix 8 and a thread owns one last-stage butterfly
+// per line: slot j of thread x is element n = x/T + j*N/8 of line x%T, for all three components at once.
+// Sums: one partial per workgroup (fixed-order tree), grid ntile*by.
+template <int N> struct ZuSched;
+template <> struct ZuSched<512> { using L = RList<8, 8, 8>; };
+template <> struct ZuSched<256> { using L = RList<8, 4, 8>; };
+#ifndef NSL_ZU_T
+#define NSL_ZU_T 8
+#endif
+#ifndef NSL_ZU_GROUP
+#define NSL_ZU_GROUP 1
+#endif
+template <int N> struct ZuCfg {
+  static constexpr int T = NSL_ZU_T, R = 8, NB = N / R, NT = NB * T;
+  static constexpr int MINW = NT >= 512 ? 1 : 512 / NT;   // at most 256 registers (no accumulation registers)
+};
+// The last-stage slots j (elements j*N/8 .. j*N/8 + N/8 - 1) that hold a retained kz, in order.
+template <int N> struct ZuSlots {
+  static constexpr Geo g = trunc_geo<N>();
+  static constexpr bool has(int j) { return !(j * (N / 8) >= g.hy && j * (N / 8) + N / 8 - 1 < N - g.by + g.hy); }
+  static constexpr int count() {
+    int c = 0;
+    for (int j = 0; j < 8; j++) c += has(j) ? 1 : 0;
+    return c;
+  }
+  static constexpr int at(int i) {
+    for (int j = 0, c = 0; j < 8; j++)
+      if (has(j)) {
+        if (c == i) return j;
+        c++;
+      }
+    return 0;
+  }
+};
+template <int N, int STAGE>
+__global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(const double2 *__restrict__ X4, const double2 *U,
+                                                            double2 *S, double2 *__restrict__ ACC, double2 *TS,
+                                                            const Geo, const double2 *__restrict__ tw, double inv,
+                                                            double *__restrict__ partials, Ctl *__restrict__ ctl,
+                                                            size_t sb) {
+  using C = ZuCfg<N>;
+  constexpr int T = C::T, R = C::R, NB = C::NB, NT = C::NT;
+  X4 = at_slot(X4, sb); U = at_slot(U, sb); S = at_slot(S, sb); ACC = at_slot(ACC, sb); TS = at_slot(TS, sb);
+  partials = at_slot(partials, sb); ctl = at_slot(ctl, sb);
+  if (ctl->active == 0) return;
+  extern __shared__ double2 lds[];
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
+  const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, kx0 = tile * T;
+  const double2 *Ib = X4 + (size_t)cy * g.hp + kx0;
+  double2 Nl[3][R];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    auto load = [&](int n, int t) -> double2 {
+      return kx0 + t < g.hx ? Ib[((size_t)c * N + n) * plane + t] : make_double2(0.0, 0.0);
+    };
+    auto keep = [&](int, int, double2 v, int, int j) { Nl[c][j] = make_double2(v.x * inv, v.y * inv); };
+    fft_sched<N, T, NT, -1, false, Interleaved<N, T>, typename ZuSched<N>::L>(lds, tw, load, keep);
+  }
+  const double nu = ctl->nu;
+  const double dt = STAGE == 1 ? choose_dt(ctl, sqrt(__longlong_as_double((long long)ctl->u2))) : ctl->dt;
+  const int t = threadIdx.x % T, bb = threadIdx.x / T, ix = kx0 + t;
+  const double kx = (double)ix, ky = kwv(full_of(cy, g), N);
+  const double h = (ix == 0 || 2 * ix == N) ? 1.0 : 2.0;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // Tnl Pspec E Z Pal nE nZ nP
+  // slots whose element range holds a retained kz, in groups of ZG: a group's loads are unconditional (a slot
+  // with no retained kz, or a padding column kx >= hx, reads mode cz = 0 of its column and discards it) so they
+  // issue back to back; only the writes and sums are conditional
+  constexpr int ZG = NSL_ZU_GROUP, nj = ZuSlots<N>::count();
+#pragma unroll
+  for (int g0 = 0; g0 < nj; g0 += ZG) {
+    UpdIn in[ZG];
+#pragma unroll
+    for (int q = 0; q < ZG; q++) {
+      if (g0 + q >= nj) break;
+      const int n = bb + ZuSlots<N>::at(g0 + q) * NB, cz = max(compact_of(n, g), 0);
+      in[q] = upd_load<STAGE>(((size_t)cz * g.by + cy) * g.hp + ix, g.D, U, S, ACC);
+    }
+#pragma unroll
+    for (int q = 0; q < ZG; q++) {
+      if (g0 + q >= nj) break;
+      const int j = ZuSlots<N>::at(g0 + q), n = bb + j * NB, cz = compact_of(n, g);
+      if (cz < 0 || ix >= g.hx) continue;
+      const size_t m = ((size_t)cz * g.by + cy) * g.hp + ix;
+      upd_apply<STAGE>(in[q], m, g.D, kx, ky, kwv(n, N), h, Nl[0][j], Nl[1][j], Nl[2][j], S, ACC, TS, nu, dt, acc);
+    }
+  }
+  block_sum<8, NT>(acc, reinterpret_cast<double *>(lds), partials + (size_t)blockIdx.x * 8);
+}
+
 // LDS layout of the x pass (interleaved measured faster than line-major on gfx950).
 #ifndef NSL_XNL_LINEMAJOR
 #define NSL_XNL_LINEMAJOR 0

@@ -358,6 +358,73 @@ __device__ __forceinline__ bool mode_k(long long m, const Geo &g, double &kx, do
 // STAGE 3: ACC += 2K, T = S + dt K                                       (nslab.js:320)
 // STAGE 4: S += dt/6 (ACC + K), and E, Z, P of the new S                  (nslab.js:322-325)
 // K = P[NL] - nu k^2 U (nslab.js:235, 240), with the rhs sums of nslab.js:238-244 into partials[block][8].
+// The update of one compact mode m, in two halves so a caller can issue the loads of several modes before it
+// applies any: upd_load reads the mode's state operands (U; S, which is U itself at stage 1; ACC from stage 2 on),
+// upd_apply projects the nonlinear term n0..n2 as the z-forward produced it, forms K, writes the stage's outputs and
+// adds the mode's rhs sums (wavenumbers kx, ky, kz, Hermitian weight h) to acc. Shared by k_update and the fused
+// z-forward.
+struct UpdIn {
+  double2 u[3], s[3], a[3];
+};
+template <int STAGE>
+__device__ __forceinline__ UpdIn upd_load(size_t m, long long D, const double2 *U, const double2 *S,
+                                          const double2 *ACC) {
+  UpdIn r;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    r.u[c] = U[c * D + m];
+    r.s[c] = STAGE == 1 ? r.u[c] : S[c * D + m];   // stage 1 evaluates the rhs at S itself
+    r.a[c] = STAGE == 1 ? make_double2(0.0, 0.0) : ACC[c * D + m];
+  }
+  return r;
+}
+template <int STAGE>
+__device__ __forceinline__ void upd_apply(const UpdIn &in, size_t m, long long D, double kx, double ky, double kz,
+                                          double h, double2 n0, double2 n1, double2 n2, double2 *S, double2 *ACC,
+                                          double2 *TS, double nu, double dt, double (&acc)[8]) {
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == 0) {
+    n0 = n1 = n2 = make_double2(0.0, 0.0);
+  } else {   // project (nslab.js:194-202)
+    const double dr = (kx * n0.x + ky * n1.x + kz * n2.x) / k2, di = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
+    n0.x -= kx * dr; n0.y -= kx * di; n1.x -= ky * dr; n1.y -= ky * di; n2.x -= kz * dr; n2.y -= kz * di;
+  }
+  const double2 u0 = in.u[0], u1 = in.u[1], u2 = in.u[2];
+  double dot = 0, e = 0;
+  dot += u0.x * n0.x + u0.y * n0.y; e += u0.x * u0.x + u0.y * u0.y;
+  dot += u1.x * n1.x + u1.y * n1.y; e += u1.x * u1.x + u1.y * u1.y;
+  dot += u2.x * n2.x + u2.y * n2.y; e += u2.x * u2.x + u2.y * u2.y;
+  const double nk = nu * k2;
+  const double2 K0 = make_double2(n0.x - nk * u0.x, n0.y - nk * u0.y);
+  const double2 K1 = make_double2(n1.x - nk * u1.x, n1.y - nk * u1.y);
+  const double2 K2 = make_double2(n2.x - nk * u2.x, n2.y - nk * u2.y);
+  acc[0] += h * dot; acc[1] += h * k2 * dot; acc[2] += h * e; acc[3] += h * k2 * e; acc[4] += h * k2 * k2 * e;
+  const double2 s0 = in.s[0], s1 = in.s[1], s2 = in.s[2];
+  if (STAGE == 1 || STAGE == 2 || STAGE == 3) {
+    const double f = STAGE == 3 ? dt : 0.5 * dt;
+    TS[m] = make_double2(s0.x + f * K0.x, s0.y + f * K0.y);
+    TS[D + m] = make_double2(s1.x + f * K1.x, s1.y + f * K1.y);
+    TS[2 * D + m] = make_double2(s2.x + f * K2.x, s2.y + f * K2.y);
+    if (STAGE == 1) {
+      ACC[m] = K0; ACC[D + m] = K1; ACC[2 * D + m] = K2;
+    } else {
+      const double2 a0 = in.a[0], a1 = in.a[1], a2 = in.a[2];
+      ACC[m] = make_double2(a0.x + 2 * K0.x, a0.y + 2 * K0.y);
+      ACC[D + m] = make_double2(a1.x + 2 * K1.x, a1.y + 2 * K1.y);
+      ACC[2 * D + m] = make_double2(a2.x + 2 * K2.x, a2.y + 2 * K2.y);
+    }
+  } else {
+    const double f = dt / 6;
+    const double2 a0 = in.a[0], a1 = in.a[1], a2 = in.a[2];
+    const double2 r0 = make_double2(s0.x + f * (a0.x + K0.x), s0.y + f * (a0.y + K0.y));
+    const double2 r1 = make_double2(s1.x + f * (a1.x + K1.x), s1.y + f * (a1.y + K1.y));
+    const double2 r2 = make_double2(s2.x + f * (a2.x + K2.x), s2.y + f * (a2.y + K2.y));
+    S[m] = r0; S[D + m] = r1; S[2 * D + m] = r2;
+    const double en = h * (r0.x * r0.x + r0.y * r0.y + r1.x * r1.x + r1.y * r1.y + r2.x * r2.x + r2.y * r2.y);
+    acc[5] += en; acc[6] += k2 * en; acc[7] += k2 * k2 * en;
+  }
+}
+
 template <int STAGE>
 // U aliases S (stage 1) or TS (stages 2-4): each mode is read before it is written, by the same thread.
 __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ NL, const double2 *U,
@@ -379,49 +446,9 @@ __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ N
     if (m >= D) break;
     double kx, ky, kz;
     if (!mode_k(m, g, kx, ky, kz)) continue;
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    double2 n0 = NL[m], n1 = NL[D + m], n2 = NL[2 * D + m];
-    if (k2 == 0) {
-      n0 = n1 = n2 = make_double2(0.0, 0.0);
-    } else {   // project (nslab.js:194-202)
-      const double dr = (kx * n0.x + ky * n1.x + kz * n2.x) / k2, di = (kx * n0.y + ky * n1.y + kz * n2.y) / k2;
-      n0.x -= kx * dr; n0.y -= kx * di; n1.x -= ky * dr; n1.y -= ky * di; n2.x -= kz * dr; n2.y -= kz * di;
-    }
-    const double2 u0 = U[m], u1 = U[D + m], u2 = U[2 * D + m];
-    double dot = 0, e = 0;
-    dot += u0.x * n0.x + u0.y * n0.y; e += u0.x * u0.x + u0.y * u0.y;
-    dot += u1.x * n1.x + u1.y * n1.y; e += u1.x * u1.x + u1.y * u1.y;
-    dot += u2.x * n2.x + u2.y * n2.y; e += u2.x * u2.x + u2.y * u2.y;
-    const double nk = nu * k2;
-    const double2 K0 = make_double2(n0.x - nk * u0.x, n0.y - nk * u0.y);
-    const double2 K1 = make_double2(n1.x - nk * u1.x, n1.y - nk * u1.y);
-    const double2 K2 = make_double2(n2.x - nk * u2.x, n2.y - nk * u2.y);
     const double h = (kx == 0 || 2 * kx == g.N) ? 1.0 : 2.0;
-    acc[0] += h * dot; acc[1] += h * k2 * dot; acc[2] += h * e; acc[3] += h * k2 * e; acc[4] += h * k2 * k2 * e;
-    const double2 s0 = S[m], s1 = S[D + m], s2 = S[2 * D + m];
-    if (STAGE == 1 || STAGE == 2 || STAGE == 3) {
-      const double f = STAGE == 3 ? dt : 0.5 * dt;
-      TS[m] = make_double2(s0.x + f * K0.x, s0.y + f * K0.y);
-      TS[D + m] = make_double2(s1.x + f * K1.x, s1.y + f * K1.y);
-      TS[2 * D + m] = make_double2(s2.x + f * K2.x, s2.y + f * K2.y);
-      if (STAGE == 1) {
-        ACC[m] = K0; ACC[D + m] = K1; ACC[2 * D + m] = K2;
-      } else {
-        const double2 a0 = ACC[m], a1 = ACC[D + m], a2 = ACC[2 * D + m];
-        ACC[m] = make_double2(a0.x + 2 * K0.x, a0.y + 2 * K0.y);
-        ACC[D + m] = make_double2(a1.x + 2 * K1.x, a1.y + 2 * K1.y);
-        ACC[2 * D + m] = make_double2(a2.x + 2 * K2.x, a2.y + 2 * K2.y);
-      }
-    } else {
-      const double f = dt / 6;
-      const double2 a0 = ACC[m], a1 = ACC[D + m], a2 = ACC[2 * D + m];
-      const double2 r0 = make_double2(s0.x + f * (a0.x + K0.x), s0.y + f * (a0.y + K0.y));
-      const double2 r1 = make_double2(s1.x + f * (a1.x + K1.x), s1.y + f * (a1.y + K1.y));
-      const double2 r2 = make_double2(s2.x + f * (a2.x + K2.x), s2.y + f * (a2.y + K2.y));
-      S[m] = r0; S[D + m] = r1; S[2 * D + m] = r2;
-      const double en = h * (r0.x * r0.x + r0.y * r0.y + r1.x * r1.x + r1.y * r1.y + r2.x * r2.x + r2.y * r2.y);
-      acc[5] += en; acc[6] += k2 * en; acc[7] += k2 * k2 * en;
-    }
+    const UpdIn in = upd_load<STAGE>(m, D, U, S, ACC);
+    upd_apply<STAGE>(in, m, D, kx, ky, kz, h, NL[m], NL[D + m], NL[2 * D + m], S, ACC, TS, nu, dt, acc);
   }
   block_sum<8>(acc, scratch, partials + (size_t)blockIdx.x * 8);
 }

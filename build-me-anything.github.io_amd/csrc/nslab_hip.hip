@@ -330,7 +330,8 @@ static SlotLayout slot_layout(const Geo &g, int nblk) {
   L.S = take(D3); L.T = take(D3); L.ACC = take(D3); L.NL = take(D3);
   L.X1 = take((size_t)6 * g.N * g.by * g.hp * sizeof(double2));
   L.X2 = take((size_t)6 * g.N * g.N * g.hp * sizeof(double2));
-  L.partials = take((size_t)nblk * 8 * sizeof(double));
+  // per-block sums of the spectral sweep (nblk blocks) or of the fused z-forward (at most by * ceil(hx / 4))
+  L.partials = take((size_t)std::max(nblk, g.by * ((g.hx + 3) / 4)) * 8 * sizeof(double));
   L.pmax = take((size_t)2 * g.N * g.N * sizeof(double));   // one (max |omega|^2, max |u|^2) per x-pass workgroup
   L.ctl = take(sizeof(Ctl));
   L.recs = take(NSL_RECCAP * sizeof(StepRecDev));
@@ -446,6 +447,14 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
   TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, bt)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt)); TRY(prof_end(s));
+  int nsum = s->nblk;   // partials the finalize sums
+  if (fast_has_zu(x)) {
+    // z-forward and update in one pass: the nonlinear term stays on chip (timed as the update of this stage)
+    TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
+    TRY(fast_dispatch_zu(x, stage, s->X1, U, s->S, s->ACC, s->T, s->partials, s->ctl, bt));
+    TRY(prof_end(s));
+    nsum = fast_zu_blocks(x.g);
+  } else {
   TRY(prof_begin(s, NSL_PROF_ZFWD)); TRY(zfwd(x, s->X1, s->NL, 3, s->ctl, bt)); TRY(prof_end(s));
   dim3 g((unsigned)s->nblk, 1, bt.nz), b(NSL_NT);
   TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
@@ -457,8 +466,9 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
   }
   LAUNCH_CHECK();
   TRY(prof_end(s));
+  }
   TRY(prof_begin(s, NSL_PROF_FINALIZE));
-  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bt.nz), dim3(NSL_NT), 0, s->st, s->partials, s->nblk, fmode, rec, s->ctl, bt.sb);
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bt.nz), dim3(NSL_NT), 0, s->st, s->partials, nsum, fmode, rec, s->ctl, bt.sb);
   LAUNCH_CHECK();
   TRY(prof_end(s));
   return NSL_OK;
