@@ -39,20 +39,23 @@ def geometry(N):
     return h, b, b * b * h
 
 
-def alg_bytes_per_launch(N):
+def alg_bytes_per_launch(N, fused=False):
     """Algorithmic HBM bytes per launch of each kernel class: every operand read once and written once
-    (complex FP64 = 16 B). h = floor(N/3)+1 retained kx, b = 2 floor(N/3)+1 retained ky/kz, D = b*b*h."""
+    (complex FP64 = 16 B). h = floor(N/3)+1 retained kx, b = 2 floor(N/3)+1 retained ky/kz, D = b*b*h.
+    fused: the z-forward runs inside the update (one kernel, timed as k_update<s>): its 3 pencils come in and the
+    nonlinear term never goes to HBM."""
     h, b, D = geometry(N)
     c = 16.0
+    z = 3 * h * b * N   # z-forward input pencils
     return {
         "k_zinv": (3 * D + 6 * h * b * N) * c,           # 3 velocity fields in, u and omega (6) out
         "k_yinv": (6 * h * b * N + 6 * h * N * N) * c,   # 6 in, 6 out
         "k_xnl": (6 * h * N * N + 3 * h * N * N) * c,    # 6 in, u x omega (3) out
         "k_yfwd": (3 * h * N * N + 3 * h * b * N) * c,   # 3 in, 3 out
-        "k_zfwd": (3 * h * b * N + 3 * D) * c,           # 3 in, NL (3) out
-        "k_update<1>": 12 * D * c,                       # NL, S in; ACC, T out
-        "k_update<2|3>": 18 * D * c,                     # NL, T, S, ACC in; T, ACC out
-        "k_update<4>": 15 * D * c,                       # NL, T, S, ACC in; S out
+        "k_zfwd": (z + 3 * D) * c,                       # 3 in, NL (3) out
+        "k_update<1>": (z + 9 * D if fused else 12 * D) * c,       # (NL,) S in; ACC, T out
+        "k_update<2|3>": (z + 15 * D if fused else 18 * D) * c,    # (NL,) T, S, ACC in; T, ACC out
+        "k_update<4>": (z + 12 * D if fused else 15 * D) * c,      # (NL,) T, S, ACC in; S out
         "k_finalize": 0.0,
     }
 
@@ -90,7 +93,8 @@ def measure(N, steps, warmup, dt, label):
     capi.check(L.nsl_profile(s.h, 0))
     assert done == steps and done_p == steps, (done, done_p, steps)
     elapsed = t1 - t0
-    per = alg_bytes_per_launch(N)
+    fused = cnt[CLASSES.index("k_zfwd")] == 0   # no separate z-forward launches: the fused pass ran
+    per = alg_bytes_per_launch(N, fused)
     kern = {}
     for i, name in enumerate(CLASSES):
         if cnt[i]:
@@ -103,7 +107,7 @@ def measure(N, steps, warmup, dt, label):
     del s
     return {"label": label, "N": N, "steps": steps, "elapsed_s": elapsed, "steps_per_s": steps / elapsed,
             "ms_per_step": 1e3 * elapsed / steps, "ms_per_step_profiled": 1e3 * (tp1 - tp0) / steps,
-            "kernels": kern, "dominant": dom, "E_end": E,
+            "kernels": kern, "dominant": dom, "E_end": E, "zfwd_fused": fused,
             "step_roofline_frac": b_alg_step(N) * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
 
 
@@ -244,6 +248,7 @@ def main():
                      "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": k["alg_bytes"], "avg_launch_us": k["avg_us"]},
         "step_roofline_frac": main_r["step_roofline_frac"],
+        "zfwd_fused_with_update": main_r["zfwd_fused"],
         "ms_per_step_profiled": main_r["ms_per_step_profiled"],
         "b_alg_per_step_GB": b_alg_step(a.N) / 1e9,
         "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1), "launches": v["launches"]}

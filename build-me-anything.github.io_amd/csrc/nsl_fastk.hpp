@@ -27,6 +27,14 @@ __host__ __device__ constexpr Geo trunc_geo() {
 #endif
 template <int N> struct PassNT { static constexpr int v = N >= 512 ? NSL_PNT_LARGE : NSL_NT; };
 
+// First-stage slot j of a pencil transform along y or z reads elements j*NB .. j*NB+NB-1 (NB = N / first radix).
+// True when none of them is a retained wavenumber: the slot is zero and needs no load.
+template <int N> __device__ __forceinline__ constexpr bool zero_slot(int j) {
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int NB = N / FirstR<typename Sched<N>::L>::R;
+  return j * NB >= g.hy && j * NB + NB - 1 < N - g.by + g.hy;
+}
+
 // P1: X[o][z][cy][kx] = z-inverse of field o (MODE 0 stored, 1 u/omega, 2 gradient of gc). Grid ntile*nout*by,
 // tile fastest: neighbouring workgroups write neighbouring segments of the same rows, and the nout workgroups that
 // read the same state tile run close together (their re-reads of the state hit L2).
@@ -85,14 +93,26 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
   OR A, B;
   auto keepA = [&](int, int, double2 v, int k, int j) { A.v[k][j] = v; };
   auto keepB = [&](int, int, double2 v, int k, int j) { B.v[k][j] = v; };
+  // Below N = 512 the i kz factor multiplies the selected value (zero off the retained band) after the load's
+  // branch, so no branch holds arithmetic on a loaded value and a transform's loads stay in flight together
+  // (256^3: 118 -> 110 us). At N >= 512 that needs more than the 128-register cap and spills (993 -> 1046 us), so
+  // the product stays on the load's path there.
   auto load_c = [&](int c, bool ikz) {
-    return [=](int n, int t) -> double2 {
+    return [=](int n, int t, int, int j) -> double2 {
       const int cz = compact_of(n, g);
-      if (cz < 0 || kx0 + t >= g.hx) return make_double2(0.0, 0.0);
-      const double2 u = U[(size_t)c * g.D + ubase + (size_t)cz * plane + t];
-      if (!ikz) return u;
-      const double kz = kwv(n, N);
-      return make_double2(-kz * u.y, kz * u.x);
+      if constexpr (N >= 512) {
+        if (cz < 0 || kx0 + t >= g.hx) return make_double2(0.0, 0.0);
+        const double2 u = U[(size_t)c * g.D + ubase + (size_t)cz * plane + t];
+        if (!ikz) return u;
+        const double kz = kwv(n, N);
+        return make_double2(-kz * u.y, kz * u.x);
+      } else {
+        if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
+        const double2 u = (cz >= 0 && kx0 + t < g.hx) ? U[(size_t)c * g.D + ubase + (size_t)cz * plane + t]
+                                                      : make_double2(0.0, 0.0);
+        const double kz = kwv(n, N);
+        return ikz ? make_double2(-kz * u.y, kz * u.x) : u;
+      }
     };
   };
   // out(o, f): write f(k, j) to field o at every register slot of this thread
@@ -146,7 +166,8 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
   const double2 *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
   double2 *Ob = X2 + ((size_t)o * N + z) * N * g.hp + kx0;
-  auto load = [&](int n, int t) -> double2 {
+  auto load = [&](int n, int t, int, int j) -> double2 {
+    if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
     const int cy = compact_of(n, g);
     return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
   };
