@@ -27,6 +27,12 @@ __host__ __device__ constexpr Geo trunc_geo() {
 #endif
 template <int N> struct PassNT { static constexpr int v = N >= 512 ? NSL_PNT_LARGE : NSL_NT; };
 
+// Pencil passes store whole tiles, the padding columns kx = hx .. hp-1 included (zeros), so each 128-byte line is
+// written in full.
+#ifndef NSL_FULL_LINES
+#define NSL_FULL_LINES 1
+#endif
+
 // First-stage slot j of a pencil transform along y or z reads elements j*NB .. j*NB+NB-1 (NB = N / first radix).
 // True when none of them is a retained wavenumber: the slot is zero and needs no load.
 template <int N> __device__ __forceinline__ constexpr bool zero_slot(int j) {
@@ -83,6 +89,7 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
   using OR = OutRegs<N, T, ZinvNT<N>::v>;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -116,10 +123,10 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
     };
   };
   // out(o, f): write f(k, j) to field o at every register slot of this thread
-  auto out = [&](int o, auto f) {
+  auto out = [&](int o, auto f) {   // whole tiles, padding columns (zero inputs, zero outputs) included
 #pragma unroll
     for (int k = 0; k < OR::K; k++) {
-      if (!OR::valid(k) || kx0 + OR::line(k) >= g.hx) continue;
+      if (!OR::valid(k) || (!NSL_FULL_LINES && kx0 + OR::line(k) >= g.hx)) continue;
 #pragma unroll
       for (int j = 0; j < OR::R; j++) Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)] = f(k, j);
     }
@@ -162,6 +169,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
   const double2 *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
@@ -171,8 +179,10 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
     const int cy = compact_of(n, g);
     return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
   };
+  // every column of the tile is stored, padding kx >= hx included (its inputs are zero, so it stores zeros): whole
+  // 128-byte lines, never a partial line
   auto store = [&](int n, int t, double2 v) {
-    if (kx0 + t < g.hx) Ob[n * g.hp + t] = v;
+    if (NSL_FULL_LINES || kx0 + t < g.hx) Ob[n * g.hp + t] = v;
   };
   fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
 }
@@ -185,6 +195,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restric
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
   const double2 *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
@@ -192,9 +203,9 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restric
   auto load = [&](int n, int t) -> double2 {
     return kx0 + t < g.hx ? Ib[n * g.hp + t] : make_double2(0.0, 0.0);
   };
-  auto store = [&](int n, int t, double2 v) {
+  auto store = [&](int n, int t, double2 v) {   // whole tiles, as in the y-inverse
     const int cy = compact_of(n, g);
-    if (cy >= 0 && kx0 + t < g.hx) Ob[cy * g.hp + t] = v;
+    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx)) Ob[cy * g.hp + t] = v;
   };
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
@@ -272,6 +283,7 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
   if (ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
   const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, kx0 = tile * T;
   const double2 *Ib = X4 + (size_t)cy * g.hp + kx0;
@@ -307,8 +319,12 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
     for (int q = 0; q < ZG; q++) {
       if (g0 + q >= nj) break;
       const int j = ZuSlots<N>::at(g0 + q), n = bb + j * NB, cz = compact_of(n, g);
-      if (cz < 0 || ix >= g.hx) continue;
+      if (cz < 0) continue;
       const size_t m = ((size_t)cz * g.by + cy) * g.hp + ix;
+      if (ix >= g.hx) {   // padding column: zeros, so every 128-byte line of the outputs is written whole
+        if (NSL_FULL_LINES) upd_zero<STAGE>(m, g.D, S, ACC, TS);
+        continue;
+      }
       upd_apply<STAGE>(in[q], m, g.D, kx, ky, kwv(n, N), h, Nl[0][j], Nl[1][j], Nl[2][j], S, ACC, TS, nu, dt, acc);
     }
   }
@@ -621,19 +637,30 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
     };
     fft_sched<N, RPW, 64, -1, false, LY, SCH>(lds, twc, ld2, st2);
     __syncthreads();
+    // whole rows, padding columns hx..HP-1 included (as zeros): every 128-byte line is written in full, which
+    // the memory system retires far faster than a partial line (1474 -> 1269 us for this access pattern alone)
 #pragma unroll
-    for (int i = 0; i < (RPW * hx + 63) / 64; i++) {
+    for (int i = 0; i < (RPW * HP + 63) / 64; i++) {
       const int e = lane + i * 64;
-      if (e >= RPW * hx) continue;
-      const int t = e / hx, k = e - t * hx;
+      if (e >= RPW * HP) continue;
+      const int t = e / HP, k = e - t * HP;
       const size_t base = (size_t)(row0 + t) * HP;
-      const double2 Zk = lds[LY::at(k, t)], Zc = lds[LY::at((N - k) % N, t)];
-      if (l == 0) {   // n_x + i n_y
-        const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-        X2[base + k] = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-        X2[cs + base + k] = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
-      } else {        // n_z (imaginary part zero)
-        X2[2 * cs + base + k] = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+      double2 o0 = make_double2(0.0, 0.0), o1 = o0;
+      if (k < hx) {
+        const double2 Zk = lds[LY::at(k, t)], Zc = lds[LY::at((N - k) % N, t)];
+        if (l == 0) {   // n_x + i n_y
+          const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+          o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+          o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+        } else {        // n_z (imaginary part zero)
+          o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+        }
+      }
+      if (l == 0) {
+        X2[base + k] = o0;
+        X2[cs + base + k] = o1;
+      } else {
+        X2[2 * cs + base + k] = o0;
       }
     }
     __syncthreads();   // the next line's first stage overwrites LDS

@@ -425,6 +425,22 @@ __device__ __forceinline__ void upd_apply(const UpdIn &in, size_t m, long long D
   }
 }
 
+// Zeros where upd_apply<STAGE> writes, for a padding column kx >= hx of the compact rows (its caller writes whole
+// 128-byte lines).
+template <int STAGE>
+__device__ __forceinline__ void upd_zero(size_t m, long long D, double2 *S, double2 *ACC, double2 *TS) {
+  const double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (STAGE == 4) {
+      S[c * D + m] = z;
+    } else {
+      TS[c * D + m] = z;
+      ACC[c * D + m] = z;
+    }
+  }
+}
+
 template <int STAGE>
 // U aliases S (stage 1) or TS (stages 2-4): each mode is read before it is written, by the same thread.
 __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ NL, const double2 *U,
