@@ -219,6 +219,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
   const int plane = g.by * g.hp;
@@ -227,9 +228,9 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
   auto load = [&](int n, int t) -> double2 {
     return kx0 + t < g.hx ? Ib[(size_t)n * plane + t] : make_double2(0.0, 0.0);
   };
-  auto store = [&](int n, int t, double2 v) {
+  auto store = [&](int n, int t, double2 v) {   // whole tiles, as in the y-inverse
     const int cz = compact_of(n, g);
-    if (cz >= 0 && kx0 + t < g.hx) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
+    if (cz >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx)) Ob[(size_t)cz * plane + t] = make_double2(v.x * inv, v.y * inv);
   };
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
@@ -438,19 +439,23 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
   };
   fft<N, T2, NSL_NT, -1, true, L2>(lds, tw, ldl2, stl2);
   __syncthreads();
-  constexpr int ITEMS2 = T2 * hx, KS = (ITEMS2 + NSL_NT - 1) / NSL_NT;
+  // whole rows, padding columns hx..HP-1 as zeros (every 128-byte line written in full)
+  constexpr int ITEMS2 = T2 * HP, KS = (ITEMS2 + NSL_NT - 1) / NSL_NT;
 #pragma unroll
   for (int i = 0; i < KS; i++) {
     const int e = threadIdx.x + i * NSL_NT;
     if (!(ITEMS2 % NSL_NT == 0 || e < ITEMS2)) continue;
-    const int t = e / hx, k = e - t * hx, p = t / 3, c = t - p * 3;
+    const int t = e / HP, k = e - t * HP, p = t / 3, c = t - p * 3;
     const int pr = grp * LP + p;
     if (pr >= npairs) continue;
     const int z = pr / half, y = 2 * (pr - z * half);
-    const double2 Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
-    const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-    const double2 X = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-    const double2 Y = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+    double2 X = make_double2(0.0, 0.0), Y = X;
+    if (k < hx) {
+      const double2 Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
+      const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+      X = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+      Y = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+    }
     const size_t base = c * cs + ((size_t)z * N + y) * HP;
     X2[base + k] = X;
     X2[base + HP + k] = Y;
@@ -538,25 +543,32 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
   };
   fft<N, T2, NT, -1, true, L2>(lds, tw, ldl2, stl2);
   __syncthreads();
-  constexpr int ITEMS2 = RP * hx, KS = (ITEMS2 + NT - 1) / NT;
+  // whole rows, padding columns hx..HP-1 as zeros (every 128-byte line written in full)
+  constexpr int ITEMS2 = RP * HP, KS = (ITEMS2 + NT - 1) / NT;
 #pragma unroll
   for (int i = 0; i < KS; i++) {
     const int e = threadIdx.x + i * NT;
     if (!(ITEMS2 % NT == 0 || e < ITEMS2)) continue;
-    const int r = e / hx, k = e - r * hx;
+    const int r = e / HP, k = e - r * HP;
     if (row0 + r >= nrows) continue;
     const size_t base = (size_t)(row0 + r) * HP + k;
     const int kk = (N - k) % N;
-    {  // n_x + i n_y
-      const double2 Zk = lds[L2::at(k, 2 * r)], Zc = lds[L2::at(kk, 2 * r)];
-      const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-      X2[base] = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-      X2[cs + base] = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+    double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
+    if (k < hx) {
+      {  // n_x + i n_y
+        const double2 Zk = lds[L2::at(k, 2 * r)], Zc = lds[L2::at(kk, 2 * r)];
+        const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+        o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+        o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+      }
+      {  // n_z (imaginary part zero)
+        const double2 Zk = lds[L2::at(k, 2 * r + 1)], Zc = lds[L2::at(kk, 2 * r + 1)];
+        o2 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+      }
     }
-    {  // n_z (imaginary part zero)
-      const double2 Zk = lds[L2::at(k, 2 * r + 1)], Zc = lds[L2::at(kk, 2 * r + 1)];
-      X2[2 * cs + base] = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
-    }
+    X2[base] = o0;
+    X2[cs + base] = o1;
+    X2[2 * cs + base] = o2;
   }
   // per-workgroup maxima
   om2 = wave_max(om2); u2 = wave_max(u2);

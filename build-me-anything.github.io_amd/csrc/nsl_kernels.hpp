@@ -461,7 +461,10 @@ __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ N
     const long long m = m0 + (long long)q * NSL_NT;
     if (m >= D) break;
     double kx, ky, kz;
-    if (!mode_k(m, g, kx, ky, kz)) continue;
+    if (!mode_k(m, g, kx, ky, kz)) {   // padding column: zeros, so the outputs' 128-byte lines are written whole
+      upd_zero<STAGE>(m, D, S, ACC, TS);
+      continue;
+    }
     const double h = (kx == 0 || 2 * kx == g.N) ? 1.0 : 2.0;
     const UpdIn in = upd_load<STAGE>(m, D, U, S, ACC);
     upd_apply<STAGE>(in, m, D, kx, ky, kz, h, NL[m], NL[D + m], NL[2 * D + m], S, ACC, TS, nu, dt, acc);
