@@ -214,11 +214,27 @@ __device__ __forceinline__ void stage_twiddles(TWS &tws, int k, int base, double
   }
 }
 
+// Barrier between stages. LW = false: the workgroup's barrier. LW = true, for a workgroup of one wave: this wave's
+// LDS operations are completed (s_waitcnt lgkmcnt(0)), and an empty asm statement with a memory clobber on each side
+// keeps the compiler from moving any memory operation across (a single-thread fence does not: lanes exchange data
+// through LDS addresses the compiler sees as distinct, and it reordered them). Unlike a workgroup fence it does not
+// wait for outstanding global loads, including loads into LDS still in flight.
+template <bool LW> __device__ __forceinline__ void stage_sync() {
+  if constexpr (LW) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt and expcnt not waited on
+    asm volatile("" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One stage. S = product of the radices before it. Input either from `load(n, t)` (FIRST) or LDS; output to
 // `store(n, t, v)` (LAST) or LDS, at the layout LY's addresses.
-template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, class LY, class TWS,
-          class LD, class ST>
+template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, bool LW, class LY,
+          class TWS, class LD, class ST>
 __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &load, ST &store) {
+  static_assert(!LW || NT == 64, "wave-only barriers need a one-wave workgroup");
   constexpr int NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
   double2 v[K][R];
 #pragma unroll
@@ -237,7 +253,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
       }
     }
   }
-  if (!FIRST || SYNC_FIRST) __syncthreads();
+  if (!FIRST || SYNC_FIRST) stage_sync<LW>();
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int idx = threadIdx.x + k * NT;
@@ -266,7 +282,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
       }
     }
   }
-  if (!LAST) __syncthreads();
+  if (!LAST) stage_sync<LW>();
 }
 
 // Register footprint of the last stage: outputs land in slots (k, j) at element n = bb + j*(N/R), line t, where
@@ -282,12 +298,13 @@ template <int N, int T, int NT> struct OutRegs {
   static __device__ __forceinline__ int elem(int k, int j) { return ((int)threadIdx.x + k * NT) / T + j * S; }
 };
 
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, class TWS, class LD, class ST,
-          int R0, int... Rs>
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, bool LW = false, class TWS,
+          class LD, class ST, int R0, int... Rs>
 __device__ __forceinline__ void run_stages(double2 *lds, TWS &tw, LD &load, ST &store, RList<R0, Rs...>) {
   constexpr bool LAST = sizeof...(Rs) == 0;
-  stage<N, T, NT, R0, S, FIRST, LAST, SIGN, SYNC_FIRST, LY>(lds, tw, load, store);
-  if constexpr (!LAST) run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, S * R0, false>(lds, tw, load, store, RList<Rs...>{});
+  stage<N, T, NT, R0, S, FIRST, LAST, SIGN, SYNC_FIRST, LW, LY>(lds, tw, load, store);
+  if constexpr (!LAST)
+    run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, S * R0, false, LW>(lds, tw, load, store, RList<Rs...>{});
 }
 
 // Full transform of T lines of length N. tw: the N roots of unity (cos, sin)(2 pi k / N). SYNC_FIRST: the first
@@ -301,6 +318,11 @@ __device__ __forceinline__ void fft(double2 *lds, TWS &tw, LD &load, ST &store) 
 template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
 __device__ __forceinline__ void fft_sched(double2 *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, SCH{});
+}
+// The same for a one-wave workgroup, with wave-only barriers (stage_sync<true>).
+template <int N, int T, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
+__device__ __forceinline__ void fft_sched_wave(double2 *lds, TWS &tw, LD &load, ST &store) {
+  run_stages<N, T, 64, SIGN, SYNC_FIRST, LY, 1, true, true>(lds, tw, load, store, SCH{});
 }
 
 // ------------------------------------------------------------------------------------------------ split form

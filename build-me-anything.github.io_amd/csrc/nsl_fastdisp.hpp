@@ -23,6 +23,12 @@ static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
 #ifndef NSL_XNL_WAVE
 #define NSL_XNL_WAVE 1
 #endif
+#ifndef NSL_XNL_DMA256
+#define NSL_XNL_DMA256 1
+#endif
+#ifndef NSL_XNL_DMA
+#define NSL_XNL_DMA 1   // stage the x pass half-spectra through LDS with load-to-LDS (kf_xnl_dma)
+#endif
 #ifndef NSL_XNL_WAVE256
 #define NSL_XNL_WAVE256 1   // 256^3: two rows per wave (8 x 4 x 8); 199 us against 257 us for the row-pair form
 #endif
@@ -110,7 +116,10 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
     constexpr int RPW = 64 / (N / 8);
     const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
     nblk = N * N / RPW;
-    hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+    if (NSL_XNL_DMA && (N == 512 || NSL_XNL_DMA256))
+      hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+    else
+      hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
   } else if constexpr (N >= 512) {
     constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
     const int lds = N * 3 * RP * (int)sizeof(double2);

@@ -681,6 +681,131 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
   if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
 }
 
+// The wave-per-row x pass with its half-spectra staged through LDS by the load-to-LDS instruction: line l+1's six
+// KB are in flight while line l is transformed, instead of being requested only when its transform starts. The
+// transforms synchronise this wave only (stage_sync<true>), since a workgroup fence would wait for the staging loads
+// in flight. Same arithmetic, in the same order, as kf_xnl_wave.
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+#ifndef NSL_XNL_DMA_LW
+#define NSL_XNL_DMA_LW 1
+#endif
+__device__ __forceinline__ void wave_lds_sync() { stage_sync<NSL_XNL_DMA_LW != 0>(); }
+template <int N>
+__global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+                                                 double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  constexpr int R = 8, NB = N / R, RPW = 64 / NB;
+  static_assert(NB * RPW == 64, "first and last stage must cover a wave");
+  using SCH = typename XnlWave<N>::L;
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int hx = g.hx, HP = g.hp;
+  constexpr size_t cs = (size_t)N * N * HP;
+  using LY = LineMajor<N, RPW>;
+  constexpr int CH = (RPW * HP + 63) / 64, FB = CH * 64;   // 1 KB chunks per field of the RPW rows
+  __shared__ double2 stg[2 * FB];
+  __shared__ double2 xch[LY::SIZE];
+  X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
+  if (ctl->active == 0) return;
+  const int lane = threadIdx.x;
+  const int row0 = blockIdx.x * RPW;
+  // rows row0 .. row0+RPW-1 are contiguous in each field: element e of the block is row t = e / HP, column e % HP
+  auto stage_line = [&](int l) {
+    const double2 *fa = X2 + 2 * l * cs + (size_t)row0 * HP, *fb = X2 + (2 * l + 1) * cs + (size_t)row0 * HP;
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int e = c * 64 + lane;
+      if (e < RPW * HP) {   // lane i of chunk c lands at stg[c*64 + i]
+        __builtin_amdgcn_global_load_lds((glb_void_t *)(fa + e), (lds_void_t *)(stg + c * 64), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((glb_void_t *)(fb + e), (lds_void_t *)(stg + FB + c * 64), 16, 0, 0);
+      }
+    }
+  };
+  stage_line(0);
+  TwCache<N, RPW, 64, LY, SCH> twc;
+  twc.load(tw);
+  double2 U[3][R];
+  int t0, bb0;
+  LY::template split<NB>(lane, t0, bb0);
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    // line l's staging loads have landed (vmcnt(0), explicit: the compiler's own wait for loads into LDS was seen
+    // placed after the first read of the staged data)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    asm volatile("" ::: "memory");
+    double2 v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      v[j] = make_double2(0.0, 0.0);
+      if (j * NB >= hx && j * NB + NB - 1 <= N - hx) continue;   // zero band
+      const int n = bb0 + j * NB;
+      const bool lo = n < hx, hi = n > N - hx;
+      const int m = t0 * HP + (lo ? n : (hi ? N - n : 0));
+      const double2 A = stg[m], B = stg[FB + m];
+      const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
+      v[j] = lo || hi ? d : make_double2(0.0, 0.0);
+    }
+    wave_lds_sync();              // the staging reads are complete before the next line overwrites stg
+    if (l < 2) stage_line(l + 1);
+    auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
+    auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
+    #if NSL_XNL_DMA_LW
+    fft_sched_wave<N, RPW, 1, false, LY, SCH>(xch, twc, ldr, keep);
+#else
+    fft_sched<N, RPW, 64, 1, false, LY, SCH>(xch, twc, ldr, keep);
+#endif
+  }
+  double om2 = 0.0, u2 = 0.0;
+  double2 P[2][R];
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    const double a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
+    om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c);
+    P[0][j] = make_double2(b * rr - c * q, c * pp - a * rr);
+    P[1][j] = make_double2(a * q - b * pp, 0.0);
+  }
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+    auto ld2 = [&](int, int, int, int j) -> double2 { return P[l][j]; };
+    auto st2 = [&](int n, int t, double2 w) {
+      if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
+    };
+    #if NSL_XNL_DMA_LW
+    fft_sched_wave<N, RPW, -1, false, LY, SCH>(xch, twc, ld2, st2);
+#else
+    fft_sched<N, RPW, 64, -1, false, LY, SCH>(xch, twc, ld2, st2);
+#endif
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < (RPW * HP + 63) / 64; i++) {
+      const int e = lane + i * 64;
+      if (e >= RPW * HP) continue;
+      const int t = e / HP, k = e - t * HP;
+      const size_t base = (size_t)(row0 + t) * HP;
+      double2 o0 = make_double2(0.0, 0.0), o1 = o0;
+      if (k < hx) {
+        const double2 Zk = xch[LY::at(k, t)], Zc = xch[LY::at((N - k) % N, t)];
+        if (l == 0) {
+          const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+          o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+          o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+        } else {
+          o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+        }
+      }
+      if (l == 0) {
+        X2[base + k] = o0;
+        X2[cs + base + k] = o1;
+      } else {
+        X2[2 * cs + base + k] = o0;
+      }
+    }
+    wave_lds_sync();   // the next line's first stage overwrites xch
+  }
+  om2 = wave_max(om2); u2 = wave_max(u2);
+  if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
+}
+
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
 // workgroups, one atomic pair each (the max is exact, so the order does not matter).
 __global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl,
