@@ -23,6 +23,22 @@ static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
 #ifndef NSL_XNL_WAVE
 #define NSL_XNL_WAVE 1
 #endif
+#ifndef NSL_XNL_PGRID
+#define NSL_XNL_PGRID 0   // > 0: the staged x pass runs this many persistent waves (0: a multiple of the CU count)
+#endif
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static std::mutex mu;
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache[dev] == 0) {
+    int n = 0;
+    cache[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+  }
+  return cache[dev];
+}
 #ifndef NSL_XNL_DMA256
 #define NSL_XNL_DMA256 1
 #endif
@@ -116,6 +132,10 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
     constexpr int RPW = 64 / (N / 8);
     const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
     nblk = N * N / RPW;
+    if (NSL_XNL_DMA) {   // persistent waves: 32 per CU at 512^3, 16 at 256^3 (8192 and 4096 on MI355X)
+      const int pg = NSL_XNL_PGRID > 0 ? NSL_XNL_PGRID : device_cus() * (N == 512 ? 32 : 16);
+      if (pg < nblk) nblk = pg;
+    }
     if (NSL_XNL_DMA && (N == 512 || NSL_XNL_DMA256))
       hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
     else

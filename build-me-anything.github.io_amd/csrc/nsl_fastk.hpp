@@ -707,10 +707,9 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   const int lane = threadIdx.x;
-  const int row0 = blockIdx.x * RPW;
-  // rows row0 .. row0+RPW-1 are contiguous in each field: element e of the block is row t = e / HP, column e % HP
-  auto stage_line = [&](int l) {
-    const double2 *fa = X2 + 2 * l * cs + (size_t)row0 * HP, *fb = X2 + (2 * l + 1) * cs + (size_t)row0 * HP;
+  // rows r .. r+RPW-1 are contiguous in each field: element e of the block is row t = e / HP, column e % HP
+  auto stage_line = [&](int l, int r) {
+    const double2 *fa = X2 + 2 * l * cs + (size_t)r * HP, *fb = X2 + (2 * l + 1) * cs + (size_t)r * HP;
 #pragma unroll
     for (int c = 0; c < CH; c++) {
       const int e = c * 64 + lane;
@@ -720,12 +719,17 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
       }
     }
   };
-  stage_line(0);
+  stage_line(0, blockIdx.x * RPW);
   TwCache<N, RPW, 64, LY, SCH> twc;
   twc.load(tw);
-  double2 U[3][R];
   int t0, bb0;
   LY::template split<NB>(lane, t0, bb0);
+  double om2 = 0.0, u2 = 0.0;
+  // a grid of fewer than N*N/RPW workgroups walks the rows; the next row group's first line is staged while this
+  // one's last line and forward transforms run
+  for (int row0 = blockIdx.x * RPW; row0 < N * N; row0 += gridDim.x * RPW) {
+  const int next = row0 + gridDim.x * RPW;
+  double2 U[3][R];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
     // line l's staging loads have landed (vmcnt(0), explicit: the compiler's own wait for loads into LDS was seen
@@ -746,7 +750,8 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
       v[j] = lo || hi ? d : make_double2(0.0, 0.0);
     }
     wave_lds_sync();              // the staging reads are complete before the next line overwrites stg
-    if (l < 2) stage_line(l + 1);
+    if (l < 2) stage_line(l + 1, row0);
+    else if (next < N * N) stage_line(0, next);
     auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
     auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
     #if NSL_XNL_DMA_LW
@@ -755,7 +760,6 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
     fft_sched<N, RPW, 64, 1, false, LY, SCH>(xch, twc, ldr, keep);
 #endif
   }
-  double om2 = 0.0, u2 = 0.0;
   double2 P[2][R];
 #pragma unroll
   for (int j = 0; j < R; j++) {
@@ -801,6 +805,7 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
       }
     }
     wave_lds_sync();   // the next line's first stage overwrites xch
+  }
   }
   om2 = wave_max(om2); u2 = wave_max(u2);
   if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
