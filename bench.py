@@ -152,6 +152,53 @@ def cpu_baseline(N, dt, budget_s=20.0):
                       "thread; host %s, %d cores visible" % (N, dt, n, t_setup, el, _cpu_model(), os.cpu_count())}
 
 
+def _oracle_steps(args):
+    """Worker: one oracle solve (random IC, CFL 0.5) of size N, `steps` RK4 steps timed after its set-up."""
+    N, steps, seed = args
+    from oracle import oracle as O
+    o = O.OracleSolver(N=N, Re=1000, ic="random", cfl=0.5, icParams={"seed": seed})
+    t0 = time.perf_counter()
+    o.run(steps)
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_ensemble(cores=None, budget_s=3.0):
+    """The ensemble mix (64x32^3, 32x48^3, 24x64^3, 8x96^3, 300 steps) on all host cores: one oracle solve per core
+    at a time, as independent processes. Per N, `cores` solves run at once for a few steps (memory contention
+    included); the mix's time is then sum(count * 300 * t_step(N)) / cores."""
+    import multiprocessing as mp
+    cores = cores or max(1, min(os.cpu_count() or 1, 16))
+    mix = ((32, 64), (48, 32), (64, 24), (96, 8))
+    per = {}
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+        for N, _ in mix:
+            steps = max(1, int(budget_s / (2e-7 * N ** 3 * 3)))   # about budget_s per worker (oracle ~0.6 us/point)
+            t = pool.map(_oracle_steps, [(N, steps, 1 + i) for i in range(cores)])
+            per[N] = sum(t) / len(t)
+    total = sum(n * 300 * per[N] for N, n in mix) / cores
+    return {"solve_steps_per_s": 128 * 300 / total, "mix_s": total, "cores": cores, "kind": "port",
+            "sec_per_step": {str(N): per[N] for N in per},
+            "sample": "the C oracle, one solve per core (%d processes at once) per N, a few RK4 steps each; the "
+                      "128-solve x 300-step mix time extrapolated from those per-step times" % cores}
+
+
+def cpu_sizes(budget_s=12.0):
+    """The oracle on one core at 32^3 (config 1: TGV dt 0.01, 200 steps) and a few steps at 64^3 and 128^3."""
+    from oracle import oracle as O
+    out = {}
+    for N, steps, dt in ((32, 200, 0.01), (64, 10, 0.004), (128, 2, 0.004)):
+        o = O.OracleSolver(N=N, Re=1600, ic="tgv", dt=dt)
+        t0 = time.perf_counter()
+        o.run(steps)
+        el = time.perf_counter() - t0
+        out[str(N)] = {"steps": steps, "steps_per_s": steps / el}
+        del o
+        if sum(1.0 / v["steps_per_s"] * v["steps"] for v in out.values()) > budget_s:
+            break
+    return out
+
+
 def node_leg(args, timeout):
     """Runs js/serve.js under Node with `args`; returns its JSON object, or {"error": ...}."""
     import shutil
@@ -177,6 +224,23 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def hbm_bytes_per_step(N):
+    """rocprofv3-measured HBM bytes of one RK4 step (FETCH_SIZE + WRITE_SIZE, calibrated) from the committed
+    summary: 4 launches of each pass per step, update stages 1, 2|3, 4 once, twice, once."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p)).get(str(N), {})
+    except (ValueError, OSError):
+        return None
+    per_step = {"k_zinv": 4, "k_yinv": 4, "k_xnl": 4, "k_yfwd": 4, "k_update<1>": 1, "k_update<2|3>": 2,
+                "k_update<4>": 1}
+    if not all(k in d for k in per_step):
+        return None
+    return sum(n * d[k]["bytes_per_launch"] for k, n in per_step.items())
 
 
 def traffic_from_profile(kernel, N):
@@ -255,8 +319,17 @@ def main():
                     for n, v in main_r["kernels"].items()},
         "cpu_baseline": cpu,
     }
+    hb = hbm_bytes_per_step(a.N)
+    if hb:
+        out["hbm_bytes_per_step_measured"] = hb
+        out["hbm_bytes_per_step_vs_b_alg"] = hb / b_alg_step(a.N)
+    if not a.no_cpu:
+        out["cpu_sizes"] = cpu_sizes()
     if extra:
+        hb512 = hbm_bytes_per_step(512)
         out["n512"] = {"workload": "Taylor-Green 512^3 FP64, dt=0.002 fixed", "steps_per_s": extra["steps_per_s"],
+                       "hbm_bytes_per_step_measured": hb512,
+                       "hbm_bytes_per_step_vs_b_alg": hb512 / b_alg_step(512) if hb512 else None,
                        "ms_per_step": extra["ms_per_step"], "step_roofline_frac": extra["step_roofline_frac"],
                        "dominant": extra["dominant"],
                        "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
@@ -274,6 +347,10 @@ def main():
             "pool_equals_single": e.get("pool_equals_single"), "graphed": e["graphed"],
             "b_alg_GBs": sum(n * b_alg_step(N) for N, n in ((32, 64), (48, 32), (64, 24), (96, 8))) * 300
                          / e["run_s"] / 1e9}
+        if not a.no_cpu and "error" not in e:
+            ce = cpu_ensemble()
+            out["ensemble"]["cpu_all_cores"] = ce
+            out["ensemble"]["vs_cpu_all_cores"] = e["solve_steps_per_s"] / ce["solve_steps_per_s"]
     if not a.no_serve:
         sv = node_leg(["serve", "--requests", str(a.serve_requests), "--rate", "2", "--seed", "1",
                        "--seq-sample", "2"], 900)
