@@ -14,18 +14,6 @@
 #define NSL_LP_ELEMS 4096
 #endif
 static constexpr int fast_T(int N) { return N <= 256 ? NSL_T_SMALL : 8; }
-// x pass in row form (one row per workgroup, three complex lines per row) from N = 512 up: half the LDS of the
-// row-pair form, so twice the workgroups per CU
-#ifndef NSL_XNL_ROWS
-#define NSL_XNL_ROWS 1
-#endif
-// x pass with one wave per row at N = 512, per two rows at N = 256 (product and forward first stage in registers)
-#ifndef NSL_XNL_WAVE
-#define NSL_XNL_WAVE 1
-#endif
-#ifndef NSL_XNL_PGRID
-#define NSL_XNL_PGRID 0   // > 0: the staged x pass runs this many persistent waves (0: a multiple of the CU count)
-#endif
 // Compute units of the current device (cached per device).
 static int device_cus() {
   static std::mutex mu;
@@ -39,21 +27,13 @@ static int device_cus() {
   }
   return cache[dev];
 }
-#ifndef NSL_XNL_DMA256
-#define NSL_XNL_DMA256 1
-#endif
-#ifndef NSL_XNL_DMA
-#define NSL_XNL_DMA 1   // stage the x pass half-spectra through LDS with load-to-LDS (kf_xnl_dma)
-#endif
-#ifndef NSL_XNL_WAVE256
-#define NSL_XNL_WAVE256 1   // 256^3: two rows per wave (8 x 4 x 8); 199 us against 257 us for the row-pair form
-#endif
 static constexpr int fast_LP(int N) {
   return 6 * N * 8 <= NSL_LP_ELEMS ? 8 : 6 * N * 4 <= NSL_LP_ELEMS ? 4 : 6 * N * 2 <= NSL_LP_ELEMS ? 2 : 1;
 }
 
 #define NSL_FAST_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
+static TuneCfg tune_for(int N);
 static bool fast_setup(Xform &x) {
   switch (x.g.N) {
 #define NSL_CASE(n) case n:
@@ -62,6 +42,7 @@ static bool fast_setup(Xform &x) {
     x.fast = true;
     x.fT = fast_T(x.g.N);
     x.fLP = fast_LP(x.g.N);
+    x.tune = tune_for(x.g.N);
     return true;
     default:
       x.fast = false;
@@ -123,39 +104,72 @@ static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncom
   LAUNCH_CHECK();
   return NSL_OK;
 }
-// pmax: 2 * (workgroups of the x pass) doubles per slot, at most 2 N^2
-template <int N>
-static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
-  int nblk;
-#if NSL_XNL_ROWS
-  if constexpr ((N == 512 || (N == 256 && NSL_XNL_WAVE256)) && NSL_XNL_WAVE) {
-    constexpr int RPW = 64 / (N / 8);
-    const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
-    nblk = N * N / RPW;
-    if (NSL_XNL_DMA) {   // persistent waves: 32 per CU at 512^3, 16 at 256^3 (8192 and 4096 on MI355X)
-      const int pg = NSL_XNL_PGRID > 0 ? NSL_XNL_PGRID : device_cus() * (N == 512 ? 32 : 16);
-      if (pg < nblk) nblk = pg;
-    }
-    if (NSL_XNL_DMA && (N == 512 || NSL_XNL_DMA256))
-      hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-    else
-      hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-  } else if constexpr (N >= 512) {
-    constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
-    const int lds = N * 3 * RP * (int)sizeof(double2);
-    allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);
-    nblk = (N * N + RP - 1) / RP;
-    hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk, 1, bt.nz), dim3(NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-  } else
-#endif
-  {
-    constexpr int LP = fast_LP(N);
+// x-pass forms compiled for each N. Row pairs: N <= 384, LP row pairs per workgroup with 6*LP*N complex of LDS
+// (at most 160 KB). Rows: N >= 384. One wave per row group (plain or staged): N = 256, 512.
+enum { NSL_XP_PAIR = 1, NSL_XP_ROWS = 2, NSL_XP_WAVE = 3, NSL_XP_STAGED = 4 };
+static constexpr bool xp_pair_lp_ok(int N, int LP) { return N <= 384 && 6 * LP * N * 16 <= 160 * 1024; }
+static constexpr bool xp_has(int N, int form) {
+  return form == NSL_XP_PAIR ? N <= 384
+       : form == NSL_XP_ROWS ? N >= 384
+       : (form == NSL_XP_WAVE || form == NSL_XP_STAGED) ? (N == 256 || N == 512) : false;
+}
+static constexpr bool zu_has(int N) { return N == 256 || N == 512; }
+
+template <int N, int LP>
+static int fast_xnl_pair(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
+  if constexpr (xp_pair_lp_ok(N, LP)) {
     const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
     const long long npairs = (long long)N * (N / 2);
     allow_lds(fast::kf_xnl<N, LP>, lds);
-    nblk = (int)((npairs + LP - 1) / LP);
+    const int nblk = (int)((npairs + LP - 1) / LP);
     hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk, 1, bt.nz), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+    return nblk;
   }
+  return 0;
+}
+
+// pmax: 2 * (workgroups of the x pass) doubles per slot, at most 2 N^2
+template <int N>
+static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
+  const TuneCfg &c = x.tune;
+  int nblk = 0;
+  switch (c.xpass) {
+    case NSL_XP_WAVE:
+    case NSL_XP_STAGED:
+      if constexpr (xp_has(N, NSL_XP_WAVE)) {
+        constexpr int RPW = 64 / (N / 8);
+        nblk = N * N / RPW;
+        if (c.xpass == NSL_XP_STAGED) {
+          if (c.xwaves > 0) nblk = std::min(nblk, device_cus() * c.xwaves);   // persistent waves walk the rows
+          hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        } else {
+          const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
+          hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        }
+      }
+      break;
+    case NSL_XP_ROWS:
+      if constexpr (xp_has(N, NSL_XP_ROWS)) {
+        constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
+        const int lds = N * 3 * RP * (int)sizeof(double2);
+        allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);
+        nblk = (N * N + RP - 1) / RP;
+        hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk, 1, bt.nz), dim3(NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+      }
+      break;
+    case NSL_XP_PAIR:
+      switch (c.lp) {
+        case 1: nblk = fast_xnl_pair<N, 1>(x, X2, pmax, ctl, bt); break;
+        case 2: nblk = fast_xnl_pair<N, 2>(x, X2, pmax, ctl, bt); break;
+        case 4: nblk = fast_xnl_pair<N, 4>(x, X2, pmax, ctl, bt); break;
+        case 8: nblk = fast_xnl_pair<N, 8>(x, X2, pmax, ctl, bt); break;
+        default: break;
+      }
+      break;
+    default:
+      break;
+  }
+  if (nblk == 0) return fail(NSL_ERR_STATE, "x-pass form " + std::to_string(c.xpass) + "/" + std::to_string(c.lp) + " is not compiled for N = " + std::to_string(N));
   LAUNCH_CHECK();
   hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512), 1, bt.nz), dim3(NSL_NT), 0, x.st, (const double *)pmax, nblk, ctl, bt.sb);
   LAUNCH_CHECK();
@@ -196,11 +210,9 @@ static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl
   NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt)
 }
 
-// z-forward fused with the stage-`stage` update (kf_zfwd_upd) where it is compiled: N = 256 and 512.
-#ifndef NSL_ZU
-#define NSL_ZU 1
-#endif
-static bool fast_has_zu(const Xform &x) { return NSL_ZU && x.fast && (x.g.N == 256 || x.g.N == 512); }
+// z-forward fused with the stage-`stage` update (kf_zfwd_upd) where it is compiled (zu_has) and chosen
+// (tune.zfused).
+static bool fast_has_zu(const Xform &x) { return x.fast && zu_has(x.g.N) && x.tune.zfused; }
 // Workgroups (and sum partials) of the fused pass.
 static int fast_zu_blocks(const Geo &g) { return ((g.hx + NSL_ZU_T - 1) / NSL_ZU_T) * g.by; }
 template <int N, int STAGE>

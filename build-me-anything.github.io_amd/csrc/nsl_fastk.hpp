@@ -687,10 +687,7 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
 // in flight. Same arithmetic, in the same order, as kf_xnl_wave.
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
-#ifndef NSL_XNL_DMA_LW
-#define NSL_XNL_DMA_LW 1
-#endif
-__device__ __forceinline__ void wave_lds_sync() { stage_sync<NSL_XNL_DMA_LW != 0>(); }
+__device__ __forceinline__ void wave_lds_sync() { stage_sync<true>(); }
 template <int N>
 __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
                                                  double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
@@ -754,11 +751,7 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
     else if (next < N * N) stage_line(0, next);
     auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
     auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
-    #if NSL_XNL_DMA_LW
     fft_sched_wave<N, RPW, 1, false, LY, SCH>(xch, twc, ldr, keep);
-#else
-    fft_sched<N, RPW, 64, 1, false, LY, SCH>(xch, twc, ldr, keep);
-#endif
   }
   double2 P[2][R];
 #pragma unroll
@@ -774,11 +767,7 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
     auto st2 = [&](int n, int t, double2 w) {
       if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
     };
-    #if NSL_XNL_DMA_LW
     fft_sched_wave<N, RPW, -1, false, LY, SCH>(xch, twc, ld2, st2);
-#else
-    fft_sched<N, RPW, 64, -1, false, LY, SCH>(xch, twc, ld2, st2);
-#endif
     wave_lds_sync();
 #pragma unroll
     for (int i = 0; i < (RPW * HP + 63) / 64; i++) {

@@ -134,6 +134,13 @@ static void allow_lds(K kernel, int bytes) {
 
 // ------------------------------------------------------------------------------------------------ transform set
 // A geometry + plan + tiling + work buffers, able to run the passes on a stream.
+// Kernel forms of one N, the autotuner's choice (nsl_tune.hpp): x-pass form (NSL_XP_*), row pairs per workgroup
+// of the row-pair form, persistent waves per CU of the staged form (0: one row group per workgroup), and whether
+// the z-forward runs fused with the update.
+struct TuneCfg {
+  int xpass = 0, lp = 0, xwaves = 0, zfused = 0;
+};
+
 struct Xform {
   Geo g;
   HostPlan hp;
@@ -142,6 +149,7 @@ struct Xform {
   // size-specialised engine (nsl_fastk.hpp) when N has a compiled schedule
   bool fast = false;
   int fT = 0, fLP = 0;
+  TuneCfg tune;             // fixed when the transform set is made: every solve of this set runs the same kernels
   double2 *twN = nullptr;   // (cos, sin)(2 pi k / N), k < N
 };
 
@@ -309,6 +317,7 @@ static void xform_free(Xform &x) {
 }
 
 #include "nsl_fastdisp.hpp"
+#include "nsl_tune.hpp"
 
 // ------------------------------------------------------------------------------------------------ solver
 #define NSL_RECCAP 64
@@ -1234,6 +1243,45 @@ int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
 }
 
 int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out) { return pool_create_impl(N, slots, out); }
+
+static bool fast_size(int N) {
+  switch (N) {
+#define NSL_CASE(n) case n:
+    NSL_FAST_SIZES(NSL_CASE)
+#undef NSL_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+static nsl_tune to_abi(const TuneCfg &c) { return nsl_tune{c.xpass, c.lp, c.xwaves, c.zfused}; }
+
+int nsl_tune_get(int32_t N, nsl_tune *out) {
+  if (!fast_size(N)) return fail(NSL_ERR_ARG, "no size-specialised kernels for N = " + std::to_string(N));
+  if (out) *out = to_abi(tune_for(N));
+  return NSL_OK;
+}
+
+int nsl_tune_set(int32_t N, const nsl_tune *cfg) {
+  if (!fast_size(N)) return fail(NSL_ERR_ARG, "no size-specialised kernels for N = " + std::to_string(N));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (cfg == nullptr) {
+    g_tune_over.erase(N);
+    return NSL_OK;
+  }
+  TuneCfg c;
+  c.xpass = cfg->xpass; c.lp = cfg->lp; c.xwaves = cfg->xwaves; c.zfused = cfg->zfused;
+  if (!tune_valid(N, c)) return fail(NSL_ERR_ARG, "kernel forms not compiled for N = " + std::to_string(N));
+  g_tune_over[N] = c;
+  return NSL_OK;
+}
+
+int nsl_tune_candidates(int32_t N, nsl_tune *out, int32_t cap) {
+  if (!fast_size(N)) return fail(NSL_ERR_ARG, "no size-specialised kernels for N = " + std::to_string(N));
+  const std::vector<TuneCfg> v = tune_candidates(N);
+  if (out) for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = to_abi(v[i]);
+  return (int)v.size();
+}
 
 void nsl_pool_destroy(nsl_pool *p) { pool_free(p); }
 

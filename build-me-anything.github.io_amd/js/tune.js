@@ -1,0 +1,110 @@
+'use strict';
+/*
+ * Autotuner of the kernel forms per grid size. For each N it times every compiled form (x-pass form and its tile,
+ * persistent waves of the staged x pass, z-forward fused with the update or not; nsl_tune_candidates) on this
+ * device, checks that each gives the same step as the current form, and writes the table the library compiles in
+ * (csrc/nsl_tune_table.inc) plus a JSON report. A form replaces the current one only when it is faster by more
+ * than the noise margin, so re-running on the same device keeps the table stable.
+ *
+ * The table is a build input: solvers then pick their forms by N alone (a solve gives the same bits alone and in
+ * any pool). NSLAB_TUNE=<file> ("N xpass lp xwaves zfused" per line) overrides it at run time without a rebuild.
+ *
+ *   node js/tune.js [--sizes 32,48,64,96,128,192,256,384,512] [--budget 0.6] [--margin 0.02]
+ *                   [--out csrc/nsl_tune_table.inc] [--report tune_report.json] [--dry]
+ */
+const fs = require('fs');
+const path = require('path');
+const addon = require(path.join(__dirname, '..', 'lib', 'nslab_napi.node'));
+
+const FORM = { 1: 'row pairs', 2: 'rows', 3: 'wave', 4: 'staged wave' };
+const REC = 21;   // doubles per step record of addon.run: t dt step ebal zbal last[8] d0[8]
+
+function args() {
+  const a = { sizes: [32, 48, 64, 96, 128, 192, 256, 384, 512], budget: 0.6, margin: 0.02, dry: false,
+    out: path.join(__dirname, '..', 'csrc', 'nsl_tune_table.inc'), report: null };
+  const v = process.argv.slice(2);
+  for (let i = 0; i < v.length; i++) {
+    const k = v[i];
+    if (k === '--sizes') a.sizes = v[++i].split(',').map(Number);
+    else if (k === '--budget') a.budget = +v[++i];
+    else if (k === '--margin') a.margin = +v[++i];
+    else if (k === '--out') a.out = v[++i];
+    else if (k === '--report') a.report = v[++i];
+    else if (k === '--dry') a.dry = true;
+    else throw new Error('unknown argument ' + k);
+  }
+  return a;
+}
+
+const now = () => Number(process.hrtime.bigint()) / 1e9;
+const key = (c) => c.join(',');
+const describe = (c) => FORM[c[0]] + (c[0] === 1 ? ' x' + c[1] : '') + (c[0] === 4 ? ' ' + (c[2] ? c[2] + '/CU' : 'one group per workgroup') : '') + (c[3] ? ', fused update' : '');
+
+// Steps/s of the forms `cfg` at N: TGV, nu = 1/1600, fixed dt; three untimed steps (their energy is the
+// same-result check), then about `budget` seconds.
+function timeForm(N, cfg, budget) {
+  addon.tuneSet(N, cfg[0], cfg[1], cfg[2], cfg[3]);
+  const h = addon.create(N, 0, 1 / 1600, 0.4, 1e-3, 0, 0, 0, 0);
+  try {
+    const v = addon.run(h, 3, 0, 0);
+    const E3 = v[2 * REC + 5];
+    let n = 2, el = 0;
+    for (;;) {
+      const t0 = now();
+      addon.run(h, n, 0, 0);
+      el = now() - t0;
+      if (el > 0.25 * budget || n >= 4096) break;
+      n *= 2;
+    }
+    const reps = Math.max(1, Math.round(budget / Math.max(el, 1e-6)));
+    const t0 = now();
+    const r = addon.run(h, n * reps, 0, 0);
+    const sec = now() - t0;
+    return { sps: (n * reps) / sec, E: E3, steps: r.length / REC };
+  } finally {
+    addon.destroy(h);
+  }
+}
+
+function tuneN(N, a) {
+  const cur = addon.tuneGet(N);
+  const cands = addon.tuneCandidates(N);
+  const rows = [];
+  let ref = null;
+  // the current form first: its energy is the reference for the same-result check
+  const order = [cur].concat(cands.filter((c) => key(c) !== key(cur)));
+  for (const c of order) {
+    const m = timeForm(N, c, a.budget);
+    if (ref === null) ref = m;
+    const same = Math.abs(m.E - ref.E) / Math.abs(ref.E);
+    rows.push({ form: c, what: describe(c), steps_per_s: m.sps, rel_diff_E: same });
+    if (!(same < 1e-11)) throw new Error(`N=${N} form ${describe(c)}: energy after 3 steps differs by ${same}`);
+  }
+  addon.tuneSet(N);   // drop the override
+  const best = rows.reduce((b, r) => (r.steps_per_s > b.steps_per_s ? r : b), rows[0]);
+  const keep = best.steps_per_s <= rows[0].steps_per_s * (1 + a.margin);
+  const pick = keep ? rows[0] : best;
+  return { N, current: describe(cur), pick: pick.form, pick_what: pick.what, kept_current: keep,
+    gain: pick.steps_per_s / rows[0].steps_per_s, candidates: rows };
+}
+
+function writeTable(file, results) {
+  const lines = ['// {N, xpass, lp, xwaves, zfused}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4).'];
+  for (const r of results) lines.push(`{${r.N}, ${r.pick.join(', ')}},   // ${r.pick_what}`);
+  fs.writeFileSync(file, lines.join('\n') + '\n');
+}
+
+if (require.main === module) {
+  const a = args();
+  const results = [];
+  for (const N of a.sizes) {
+    const r = tuneN(N, a);
+    results.push(r);
+    process.stdout.write(`N=${N}: ${r.pick_what}${r.kept_current ? ' (current kept)' : ''}, x${r.gain.toFixed(3)}; ` +
+      r.candidates.map((c) => `${c.what} ${c.steps_per_s.toFixed(1)}`).join(' | ') + '\n');
+  }
+  if (!a.dry) writeTable(a.out, results);
+  if (a.report) fs.writeFileSync(a.report, JSON.stringify({ device: addon.version(), results }, null, 1));
+}
+
+module.exports = { tuneN, timeForm, writeTable };

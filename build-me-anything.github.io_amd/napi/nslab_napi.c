@@ -54,15 +54,43 @@ static napi_value new_f64arr(napi_env env, size_t n, double **data) {
   return ta;
 }
 
+/* A JS solver handle: the solver, and whether the handle owns it (a pool slot view does not). destroy(h) frees an
+ * owned solver at once (its device memory with it); the finaliser frees whatever is left when h is collected. */
+typedef struct {
+  nsl_solver *s;
+  int owned;
+} solver_ref;
+
 static void finalize_solver(napi_env env, void *data, void *hint) {
   (void)env; (void)hint;
-  nsl_destroy((nsl_solver *)data);
+  solver_ref *r = (solver_ref *)data;
+  if (r->owned && r->s) nsl_destroy(r->s);
+  free(r);
+}
+
+static napi_value wrap_solver(napi_env env, nsl_solver *s, int owned) {
+  solver_ref *r = (solver_ref *)malloc(sizeof *r);
+  if (!r) { if (owned) nsl_destroy(s); napi_throw_error(env, NULL, "out of host memory"); return NULL; }
+  r->s = s;
+  r->owned = owned;
+  napi_value ext;
+  if (napi_create_external(env, r, finalize_solver, NULL, &ext) != napi_ok) {
+    finalize_solver(env, r, NULL);
+    napi_throw_error(env, NULL, "N-API call failed: napi_create_external");
+    return NULL;
+  }
+  return ext;
+}
+
+static solver_ref *get_ref(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  return (solver_ref *)p;
 }
 
 static nsl_solver *get_solver(napi_env env, napi_value v) {
-  void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
-  return (nsl_solver *)p;
+  solver_ref *r = get_ref(env, v);
+  return r ? r->s : NULL;
 }
 
 #define SOLVER(idx)                                                         \
@@ -86,9 +114,17 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   nsl_solver *s = NULL;
   int rc = nsl_create(&o, &s);
   if (rc < 0) return throw_status(env, rc);
-  napi_value ext;
-  CALL(env, napi_create_external(env, s, finalize_solver, NULL, &ext));
-  return ext;
+  return wrap_solver(env, s, 1);
+}
+
+/* destroy(h): frees the solver now (a pool slot view is left to its pool); later calls on h throw */
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  solver_ref *r = get_ref(env, argv[0]);
+  if (!r) { napi_throw_type_error(env, NULL, "not a solver handle"); return NULL; }
+  if (r->owned && r->s) nsl_destroy(r->s);
+  r->s = NULL;
+  return NULL;
 }
 
 static napi_value js_set_ic(napi_env env, napi_callback_info info) {
@@ -451,10 +487,10 @@ static napi_value js_pool_admit(napi_env env, napi_callback_info info) {
   nsl_solver *v = NULL;
   rc = nsl_pool_slot(pl, slot, &v);
   if (rc < 0) return throw_status(env, rc);
-  napi_value arr, sv, ext;
+  napi_value arr, sv, ext = wrap_solver(env, v, 0);
+  if (ext == NULL) return NULL;
   CALL(env, napi_create_array_with_length(env, 2, &arr));
   CALL(env, napi_create_int32(env, slot, &sv));
-  CALL(env, napi_create_external(env, v, NULL, NULL, &ext));
   CALL(env, napi_set_element(env, arr, 0, sv));
   CALL(env, napi_set_element(env, arr, 1, ext));
   return arr;
@@ -553,6 +589,65 @@ static napi_value js_pool_release(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* tune forms as [xpass, lp, xwaves, zfused] */
+static napi_value tune_arr(napi_env env, const nsl_tune *t) {
+  napi_value arr;
+  const int32_t v4[4] = {t->xpass, t->lp, t->xwaves, t->zfused};
+  CALL(env, napi_create_array_with_length(env, 4, &arr));
+  for (uint32_t i = 0; i < 4; i++) {
+    napi_value v;
+    CALL(env, napi_create_int32(env, v4[i], &v));
+    CALL(env, napi_set_element(env, arr, i, v));
+  }
+  return arr;
+}
+
+/* tuneGet(N) -> [xpass, lp, xwaves, zfused] */
+static napi_value js_tune_get(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  int64_t N = 0;
+  get_i64(env, argv[0], &N);
+  nsl_tune t;
+  int rc = nsl_tune_get((int32_t)N, &t);
+  if (rc < 0) return throw_status(env, rc);
+  return tune_arr(env, &t);
+}
+
+/* tuneSet(N, xpass, lp, xwaves, zfused), or tuneSet(N) to drop the override */
+static napi_value js_tune_set(napi_env env, napi_callback_info info) {
+  ARGS(5);
+  int64_t a[5] = {0, 0, 0, 0, 0};
+  for (size_t i = 0; i < argc && i < 5; i++) get_i64(env, argv[i], &a[i]);
+  int rc;
+  if (argc < 5) {
+    rc = nsl_tune_set((int32_t)a[0], NULL);
+  } else {
+    nsl_tune t = {(int32_t)a[1], (int32_t)a[2], (int32_t)a[3], (int32_t)a[4]};
+    rc = nsl_tune_set((int32_t)a[0], &t);
+  }
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+/* tuneCandidates(N) -> [[xpass, lp, xwaves, zfused], ...] */
+static napi_value js_tune_candidates(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  int64_t N = 0;
+  get_i64(env, argv[0], &N);
+  nsl_tune t[128];
+  int k = nsl_tune_candidates((int32_t)N, t, 128);
+  if (k < 0) return throw_status(env, k);
+  if (k > 128) k = 128;
+  napi_value arr;
+  CALL(env, napi_create_array_with_length(env, (size_t)k, &arr));
+  for (int i = 0; i < k; i++) {
+    napi_value e = tune_arr(env, &t[i]);
+    if (e == NULL) return NULL;
+    CALL(env, napi_set_element(env, arr, (uint32_t)i, e));
+  }
+  return arr;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value v;
@@ -562,7 +657,7 @@ static napi_value js_version(napi_env env, napi_callback_info info) {
 
 static napi_value Init(napi_env env, napi_value exports) {
   static const struct { const char *name; napi_callback cb; } fns[] = {
-    {"create", js_create}, {"setIC", js_set_ic}, {"run", js_run}, {"last", js_last},
+    {"create", js_create}, {"destroy", js_destroy}, {"setIC", js_set_ic}, {"run", js_run}, {"last", js_last},
     {"setClock", js_set_clock}, {"getClock", js_get_clock}, {"norms", js_norms},
     {"getState", js_get_state}, {"setState", js_set_state}, {"diagnose", js_diagnose},
     {"spectrum", js_spectrum}, {"interpMax", js_interp_max}, {"field", js_field},
@@ -573,6 +668,7 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"poolInfo", js_pool_info}, {"poolAdmit", js_pool_admit}, {"poolSetTarget", js_pool_set_target},
     {"poolLaunch", js_pool_launch}, {"poolCollect", js_pool_collect}, {"poolRelease", js_pool_release},
     {"deviceMemory", js_device_memory}, {"poolReady", js_pool_ready},
+    {"tuneGet", js_tune_get}, {"tuneSet", js_tune_set}, {"tuneCandidates", js_tune_candidates},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

@@ -177,6 +177,25 @@ int nsl_spectral_inverse(int32_t N, const double *re, const double *im, double *
 /* factorize(N) (nslab.js:28-35): radices into f[0..return), or NSL_ERR_ARG with the reference's message. */
 int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
 
+/* ---- Kernel forms per grid size (autotuner). No counterpart in the reference: it has one code path; the
+ * north star asks for tile and radix schedules picked per N by a project-written autotuner (js/tune.js). A
+ * solver's forms depend on N only and are fixed when it (or its pool) is created, so a solve gives the same bits
+ * alone and in any pool.
+ *   xpass : x-pass form, 1 row pairs (16x16-style schedules, lp row pairs per workgroup), 2 one row per workgroup,
+ *           3 one wave per row group (8x8x8 at 512, 8x4x8 at 256), 4 the same with its input staged through LDS
+ *   lp    : row pairs per workgroup of form 1 (1, 2, 4, 8)
+ *   xwaves: persistent waves per CU of form 4 (0: one row group per workgroup)
+ *   zfused: 1 = z-forward fused with the spectral update (N = 256, 512), 0 = two kernels */
+typedef struct {
+  int32_t xpass, lp, xwaves, zfused;
+} nsl_tune;
+/* The forms solvers of this N use: an nsl_tune_set override, else the NSLAB_TUNE file, else the tuned table. */
+int nsl_tune_get(int32_t N, nsl_tune *out);
+/* Override for solvers and pools created afterwards (cfg == NULL drops it); NSL_ERR_ARG if not compiled for N. */
+int nsl_tune_set(int32_t N, const nsl_tune *cfg);
+/* Every compiled form of N into out[0..min(return, cap)); returns the count. */
+int nsl_tune_candidates(int32_t N, nsl_tune *out, int32_t cap);
+
 /* Text of the last error on this thread ("" if none). */
 const char *nsl_last_error(void);
 /* Version string of the library. */

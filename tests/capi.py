@@ -29,6 +29,13 @@ class RhsDiag(ctypes.Structure):
         return {k: getattr(self, k) for k in LAST_KEYS}
 
 
+class Tune(ctypes.Structure):
+    _fields_ = [("xpass", ctypes.c_int32), ("lp", ctypes.c_int32), ("xwaves", ctypes.c_int32), ("zfused", ctypes.c_int32)]
+
+    def astuple(self):
+        return (self.xpass, self.lp, self.xwaves, self.zfused)
+
+
 class StepRec(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("step", ctypes.c_int64), ("ebal", ctypes.c_double),
                 ("zbal", ctypes.c_double), ("last", RhsDiag), ("d0", RhsDiag)]
@@ -87,6 +94,9 @@ def lib():
         L.nsl_pool_ready.argtypes = [vp]
         L.nsl_pool_collect.argtypes = [vp, ctypes.POINTER(StepRec), ctypes.c_int32, i64p, i32p]
         L.nsl_pool_release.argtypes = [vp, ctypes.c_int32]
+        L.nsl_tune_get.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
+        L.nsl_tune_set.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
+        L.nsl_tune_candidates.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32]
         L.nsl_last_error.restype = ctypes.c_char_p
         L.nsl_version.restype = ctypes.c_char_p
         _lib = L
@@ -301,6 +311,28 @@ def spectral_inverse(N, re, im):
     f = np.zeros(N ** 3)
     check(lib().nsl_spectral_inverse(N, _p(re), _p(im), _p(f)))
     return f
+
+
+def tune_get(N):
+    t = Tune()
+    check(lib().nsl_tune_get(N, ctypes.byref(t)))
+    return t.astuple()
+
+
+def tune_set(N, cfg=None):
+    """Override the kernel forms of N for solvers created afterwards; None drops the override."""
+    if cfg is None:
+        check(lib().nsl_tune_set(N, None))
+    else:
+        t = Tune(*cfg)
+        check(lib().nsl_tune_set(N, ctypes.byref(t)))
+
+
+def tune_candidates(N):
+    n = check(lib().nsl_tune_candidates(N, None, 0))
+    arr = (Tune * max(n, 1))()
+    check(lib().nsl_tune_candidates(N, arr, n))
+    return [arr[i].astuple() for i in range(n)]
 
 
 def factorize(N):

@@ -51,3 +51,36 @@ def test_pool_slot_bytes_cover_the_solver_buffers():
         assert big <= got <= big * 1.05 + (1 << 20), N
         assert got % 4096 == 0 and got > prev
         prev = got
+
+
+def test_tune_table_and_overrides_without_gpu():
+    """The kernel-form table: every size has a compiled form, overrides are checked against what is compiled for
+    that size, and dropping an override returns to the table."""
+    table = {}
+    for line in open(capi.ROOT + "/build-me-anything.github.io_amd/csrc/nsl_tune_table.inc"):
+        m = re.match(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", line)
+        if m and int(m.group(1)) > 0:
+            table[int(m.group(1))] = tuple(int(v) for v in m.groups()[1:])
+    for N in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
+        cur = capi.tune_get(N)
+        cands = capi.tune_candidates(N)
+        assert cur in cands, (N, cur)
+        if N in table:
+            assert cur == table[N], N
+    assert len(capi.tune_candidates(256)) == 18 and len(capi.tune_candidates(1024)) == 1
+    # forms not compiled for a size are refused
+    for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (128, (1, 2, 0, 1)), (512, (1, 1, 0, 0))):
+        try:
+            capi.tune_set(N, bad)
+            raise AssertionError("expected an error for %s at %d" % (bad, N))
+        except capi.NslError as e:
+            assert "not compiled" in str(e)
+    try:
+        capi.tune_get(100)
+        raise AssertionError("expected an error")
+    except capi.NslError:
+        pass
+    capi.tune_set(64, (1, 8, 0, 0))
+    assert capi.tune_get(64) == (1, 8, 0, 0)
+    capi.tune_set(64)
+    assert capi.tune_get(64) == table[64]

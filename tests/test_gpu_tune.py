@@ -1,0 +1,69 @@
+"""Every compiled kernel form of a size gives the tuned form's step: the autotuner may pick any of them.
+
+For N = 64 (row-pair tiles 1, 2, 4, 8), 256 (row pairs, wave, staged wave with and without persistent waves, fused
+and unfused z-forward) and 512 (rows, wave, staged, fused or not), a random-IC solve (CFL 0.5) run for 3 steps under
+each form matches the same solve under the table's form to 1e-13 relative in the state and the series. The forms
+differ only in the radix factorisation and tiling of the same transforms, so the expected difference is roundoff
+(a few 1e-16). Also: a pool created under an override uses it for every slot, and the slot stays bit-identical to a
+single solve created under the same override."""
+import numpy as np
+import pytest
+
+import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def run3(N, cfg):
+    capi.tune_set(N, cfg)
+    try:
+        s = capi.Solver(N=N, Re=1000, ic="random", cfl=0.5)
+        s.run(3)
+        st = s.state()[:, ::97].copy()
+        E = np.array(s.series["E"])
+        s.close()
+        return st, E
+    finally:
+        capi.tune_set(N)
+
+
+@pytest.mark.parametrize("N", [64, 256, 512])
+def test_every_form_matches_the_tuned_form(N):
+    ref_cfg = capi.tune_get(N)
+    ref, Eref = run3(N, ref_cfg)
+    forms = capi.tune_candidates(N)
+    if N == 512:   # the persistent-wave counts share one kernel: check one besides the tuned one
+        forms = [f for f in forms if f[0] != 4 or f[2] in (0, 16)]
+    for cfg in forms:
+        if cfg == ref_cfg:
+            continue
+        st, E = run3(N, cfg)
+        rel = float(np.max(np.abs(st - ref)) / np.max(np.abs(ref)))
+        assert rel < 1e-13, (N, cfg, rel)
+        assert np.max(np.abs(E - Eref) / Eref) < 1e-13, (N, cfg)
+
+
+def test_pool_uses_the_override_and_matches_a_single_solve():
+    N, cfg = 64, (1, 8, 0, 0)
+    assert capi.tune_get(N) != cfg
+    capi.tune_set(N, cfg)
+    try:
+        s = capi.Solver(N=N, Re=900, ic="random", cfl=0.5, icParams={"seed": 5})
+        s.run(4)
+        a = s.state()
+        s.close()
+        pool = capi.Pool(N, 2)
+        m = pool.admit(Re=900, ic="random", cfl=0.5, icParams={"seed": 5})
+        pool.set_target(m, 4)
+        pool.run_all()
+        b = m.state()
+        pool.close()
+        assert np.array_equal(a, b)
+    finally:
+        capi.tune_set(N)
+    # the table's form gives the same step to roundoff (the row-pair tile changes no arithmetic at all)
+    s = capi.Solver(N=N, Re=900, ic="random", cfl=0.5, icParams={"seed": 5})
+    s.run(4)
+    c = s.state()
+    s.close()
+    assert float(np.max(np.abs(a - c)) / np.max(np.abs(c))) < 1e-13

@@ -72,3 +72,23 @@ def test_parity_gate_grades_series():
           "console.log([r1, r2, r3, r4].join(' '));") % gate
     r = subprocess.run([node, "-e", js], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout.strip() == "PASS WARN FAIL FAIL", r.stdout + r.stderr
+
+
+@needs_node
+@pytest.mark.gpu
+def test_autotuner_runs_and_checks_every_form(tmp_path):
+    """js/tune.js times every compiled form at 32^3 and 256^3 (briefly), checks each against the current form's
+    energy after 3 steps, and reports a pick among the candidates; --dry leaves the compiled-in table alone."""
+    import json
+    script = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "tune.js")
+    rep = str(tmp_path / "tune.json")
+    r = subprocess.run([node, script, "--sizes", "32,256", "--budget", "0.05", "--dry", "--report", rep],
+                       capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    res = json.load(open(rep))["results"]
+    assert [x["N"] for x in res] == [32, 256]
+    for x in res:
+        forms = [c["form"] for c in x["candidates"]]
+        assert x["pick"] in forms and len(forms) == (4 if x["N"] == 32 else 18)
+        assert all(c["rel_diff_E"] < 1e-11 and c["steps_per_s"] > 0 for c in x["candidates"])
