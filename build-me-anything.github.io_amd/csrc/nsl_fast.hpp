@@ -214,15 +214,17 @@ __device__ __forceinline__ void stage_twiddles(TWS &tws, int k, int base, double
   }
 }
 
-// Barrier between stages. LW = false: the workgroup's barrier. LW = true, for a workgroup of one wave: this wave's
-// LDS operations are completed (s_waitcnt lgkmcnt(0)), and an empty asm statement with a memory clobber on each side
-// keeps the compiler from moving any memory operation across (a single-thread fence does not: lanes exchange data
-// through LDS addresses the compiler sees as distinct, and it reordered them). Unlike a workgroup fence it does not
-// wait for outstanding global loads, including loads into LDS still in flight.
-template <bool LW> __device__ __forceinline__ void stage_sync() {
+// Barrier between stages. LW = false: the workgroup's barrier. LW = true: an LDS-only barrier. Each wave completes
+// its own LDS operations (s_waitcnt lgkmcnt(0)); a workgroup of more than one wave (NT > 64) then meets at s_barrier.
+// An empty asm statement with a memory clobber on each side keeps the compiler from moving any memory operation
+// across. A single-thread fence does not: lanes exchange data through LDS addresses that the compiler sees as
+// distinct, and it reordered them. Unlike a workgroup fence, this does not wait for outstanding global loads,
+// including loads into LDS still in flight.
+template <bool LW, int NT = 64> __device__ __forceinline__ void stage_sync() {
   if constexpr (LW) {
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt and expcnt not waited on
+    if constexpr (NT > 64) __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
   } else {
     __syncthreads();
@@ -234,7 +236,6 @@ template <bool LW> __device__ __forceinline__ void stage_sync() {
 template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, bool LW, class LY,
           class TWS, class LD, class ST>
 __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &load, ST &store) {
-  static_assert(!LW || NT == 64, "wave-only barriers need a one-wave workgroup");
   constexpr int NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
   double2 v[K][R];
 #pragma unroll
@@ -253,7 +254,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
       }
     }
   }
-  if (!FIRST || SYNC_FIRST) stage_sync<LW>();
+  if (!FIRST || SYNC_FIRST) stage_sync<LW, NT>();
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int idx = threadIdx.x + k * NT;
@@ -282,7 +283,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
       }
     }
   }
-  if (!LAST) stage_sync<LW>();
+  if (!LAST) stage_sync<LW, NT>();
 }
 
 // Register footprint of the last stage: outputs land in slots (k, j) at element n = bb + j*(N/R), line t, where
@@ -319,10 +320,10 @@ template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, 
 __device__ __forceinline__ void fft_sched(double2 *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, SCH{});
 }
-// The same for a one-wave workgroup, with wave-only barriers (stage_sync<true>).
-template <int N, int T, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
-__device__ __forceinline__ void fft_sched_wave(double2 *lds, TWS &tw, LD &load, ST &store) {
-  run_stages<N, T, 64, SIGN, SYNC_FIRST, LY, 1, true, true>(lds, tw, load, store, SCH{});
+// The same with LDS-only barriers (stage_sync<true, NT>), for kernels with loads into LDS in flight.
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
+__device__ __forceinline__ void fft_sched_lds(double2 *lds, TWS &tw, LD &load, ST &store) {
+  run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true, true>(lds, tw, load, store, SCH{});
 }
 
 // ------------------------------------------------------------------------------------------------ split form

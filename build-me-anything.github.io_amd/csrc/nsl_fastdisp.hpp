@@ -8,7 +8,7 @@
 #define NSL_T_SMALL 8
 #endif
 #ifndef NSL_ZINV_UW_MAXN
-#define NSL_ZINV_UW_MAXN 768
+#define NSL_ZINV_UW_MAXN 1024
 #endif
 #ifndef NSL_LP_ELEMS
 #define NSL_LP_ELEMS 4096
@@ -111,7 +111,8 @@ static constexpr bool xp_pair_lp_ok(int N, int LP) { return N <= 384 && 6 * LP *
 static constexpr bool xp_has(int N, int form) {
   return form == NSL_XP_PAIR ? N <= 384
        : form == NSL_XP_ROWS ? N >= 384
-       : (form == NSL_XP_WAVE || form == NSL_XP_STAGED) ? (N == 256 || N == 512) : false;
+       : form == NSL_XP_WAVE ? (N == 256 || N == 512)
+       : form == NSL_XP_STAGED ? (N == 256 || N == 512 || N == 1024) : false;
 }
 static constexpr bool zu_has(int N) { return N == 256 || N == 512; }
 
@@ -134,18 +135,22 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
   const TuneCfg &c = x.tune;
   int nblk = 0;
   switch (c.xpass) {
-    case NSL_XP_WAVE:
     case NSL_XP_STAGED:
+      if constexpr (xp_has(N, NSL_XP_STAGED)) {
+        using C = fast::XnlStaged<N>;
+        int grid = N * N / C::RPW;
+        // persistent: xwaves waves per CU walk the rows
+        if (c.xwaves > 0) grid = std::min(grid, std::max(1, device_cus() * c.xwaves / C::WPR));
+        hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)grid, 1, bt.nz), dim3(C::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        nblk = grid;   // one maxima pair per workgroup
+      }
+      break;
+    case NSL_XP_WAVE:
       if constexpr (xp_has(N, NSL_XP_WAVE)) {
         constexpr int RPW = 64 / (N / 8);
         nblk = N * N / RPW;
-        if (c.xpass == NSL_XP_STAGED) {
-          if (c.xwaves > 0) nblk = std::min(nblk, device_cus() * c.xwaves);   // persistent waves walk the rows
-          hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-        } else {
-          const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
-          hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-        }
+        const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
+        hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
       }
       break;
     case NSL_XP_ROWS:
@@ -170,6 +175,7 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
       break;
   }
   if (nblk == 0) return fail(NSL_ERR_STATE, "x-pass form " + std::to_string(c.xpass) + "/" + std::to_string(c.lp) + " is not compiled for N = " + std::to_string(N));
+  if (nblk > N * N) return fail(NSL_ERR_STATE, "x-pass maxima exceed their buffer");   // pmax holds N^2 pairs
   LAUNCH_CHECK();
   hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512), 1, bt.nz), dim3(NSL_NT), 0, x.st, (const double *)pmax, nblk, ctl, bt.sb);
   LAUNCH_CHECK();

@@ -79,7 +79,12 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restric
 // tile instead of six and nine. Grid ntile * 2 * by, XCD-remapped so the pair shares an L2.
 // Waves per SIMD the z-inverse must fit: at N >= 512 (one butterfly per thread) a cap of 128 registers costs no
 // spills and lets two workgroups share a CU.
-template <int N> struct ZinvMinWaves { static constexpr int v = N >= 512 ? 4 : 1; };
+// Above 512 the 8-line tile's LDS (96 KB at 768, 128 KB at 1024) allows one workgroup per CU whatever the register
+// count, so the cap is 256 registers (no spills) instead of 128.
+#ifndef NSL_ZINV_BIG_MINW
+#define NSL_ZINV_BIG_MINW 2
+#endif
+template <int N> struct ZinvMinWaves { static constexpr int v = N > 512 ? NSL_ZINV_BIG_MINW : N == 512 ? 4 : 1; };
 // Threads of the z-inverse: at N = 256 (16 x 16, 16 outputs per butterfly) 128 threads, one butterfly each.
 template <int N> struct ZinvNT { static constexpr int v = N == 256 ? 128 : PassNT<N>::v; };
 template <int N, int T>
@@ -681,18 +686,30 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
   if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
 }
 
-// The wave-per-row x pass with its half-spectra staged through LDS by the load-to-LDS instruction: line l+1's six
-// KB are in flight while line l is transformed, instead of being requested only when its transform starts. The
-// transforms synchronise this wave only (stage_sync<true>), since a workgroup fence would wait for the staging loads
-// in flight. Same arithmetic, in the same order, as kf_xnl_wave.
+// The x pass with one butterfly per thread per stage and its half-spectra staged through LDS by the load-to-LDS
+// instruction: line l+1's input is in flight while line l is transformed, and a persistent workgroup stages its next
+// row group's first line during the current one's last transforms. Rows per workgroup RPW and threads NT: 2 and 64
+// at 256^3 (8x4x8), 1 and 64 at 512^3 (8x8x8), 1 and 128 at 1024^3 (8x16x8; two waves per row). The transforms use
+// LDS-only barriers (stage_sync<true, NT>), since a workgroup fence would also wait for the staging loads in flight.
+// Same arithmetic per line as kf_xnl_wave.
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
-__device__ __forceinline__ void wave_lds_sync() { stage_sync<true>(); }
+template <> struct XnlWave<1024> { using L = RList<8, 16, 8>; };
+#ifndef NSL_XNL_BIG_MINW
+#define NSL_XNL_BIG_MINW 1
+#endif
+template <int N> struct XnlStaged {
+  static constexpr int R = 8, NB = N / R, NT = NB > 64 ? NB : 64, RPW = NT / NB, WPR = NT / 64;
+  static constexpr int MINW = NT > 64 ? NSL_XNL_BIG_MINW : 1;   // two-wave rows: at most 512 / MINW registers
+};
 template <int N>
-__global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
-                                                 double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
-  constexpr int R = 8, NB = N / R, RPW = 64 / NB;
-  static_assert(NB * RPW == 64, "first and last stage must cover a wave");
+__global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
+                                                               const double2 *__restrict__ tw,
+                                                               double *__restrict__ pmax, Ctl *__restrict__ ctl,
+                                                               size_t sb) {
+  using C = XnlStaged<N>;
+  constexpr int R = C::R, NB = C::NB, NT = C::NT, RPW = C::RPW, WPR = C::WPR;
+  static_assert(NB * RPW == NT, "first and last stage: one butterfly per thread");
   using SCH = typename XnlWave<N>::L;
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
@@ -703,36 +720,37 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
   __shared__ double2 xch[LY::SIZE];
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
-  const int lane = threadIdx.x;
-  // rows r .. r+RPW-1 are contiguous in each field: element e of the block is row t = e / HP, column e % HP
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // rows r .. r+RPW-1 are contiguous in each field: element e of the block is row t = e / HP, column e % HP. Wave w
+  // stages chunks w, w + WPR, ...; lane i of chunk c lands at stg[c*64 + i].
   auto stage_line = [&](int l, int r) {
     const double2 *fa = X2 + 2 * l * cs + (size_t)r * HP, *fb = X2 + (2 * l + 1) * cs + (size_t)r * HP;
 #pragma unroll
     for (int c = 0; c < CH; c++) {
       const int e = c * 64 + lane;
-      if (e < RPW * HP) {   // lane i of chunk c lands at stg[c*64 + i]
+      if (c % WPR == wave && e < RPW * HP) {
         __builtin_amdgcn_global_load_lds((glb_void_t *)(fa + e), (lds_void_t *)(stg + c * 64), 16, 0, 0);
         __builtin_amdgcn_global_load_lds((glb_void_t *)(fb + e), (lds_void_t *)(stg + FB + c * 64), 16, 0, 0);
       }
     }
   };
   stage_line(0, blockIdx.x * RPW);
-  TwCache<N, RPW, 64, LY, SCH> twc;
+  TwCache<N, RPW, NT, LY, SCH> twc;
   twc.load(tw);
   int t0, bb0;
-  LY::template split<NB>(lane, t0, bb0);
+  LY::template split<NB>(tid, t0, bb0);
   double om2 = 0.0, u2 = 0.0;
-  // a grid of fewer than N*N/RPW workgroups walks the rows; the next row group's first line is staged while this
-  // one's last line and forward transforms run
   for (int row0 = blockIdx.x * RPW; row0 < N * N; row0 += gridDim.x * RPW) {
   const int next = row0 + gridDim.x * RPW;
   double2 U[3][R];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
-    // line l's staging loads have landed (vmcnt(0), explicit: the compiler's own wait for loads into LDS was seen
-    // placed after the first read of the staged data)
+    // line l's staging loads have landed: vmcnt(0) in every wave (explicit: the compiler's own wait for loads into
+    // LDS was seen placed after the first read of the staged data), then the workgroup meets
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    if constexpr (WPR > 1) __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     double2 v[R];
 #pragma unroll
@@ -746,12 +764,12 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
       const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
       v[j] = lo || hi ? d : make_double2(0.0, 0.0);
     }
-    wave_lds_sync();              // the staging reads are complete before the next line overwrites stg
+    stage_sync<true, NT>();       // every staging read is complete before the next line overwrites stg
     if (l < 2) stage_line(l + 1, row0);
     else if (next < N * N) stage_line(0, next);
     auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
     auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
-    fft_sched_wave<N, RPW, 1, false, LY, SCH>(xch, twc, ldr, keep);
+    fft_sched_lds<N, RPW, NT, 1, false, LY, SCH>(xch, twc, ldr, keep);
   }
   double2 P[2][R];
 #pragma unroll
@@ -767,11 +785,12 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
     auto st2 = [&](int n, int t, double2 w) {
       if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
     };
-    fft_sched_wave<N, RPW, -1, false, LY, SCH>(xch, twc, ld2, st2);
-    wave_lds_sync();
+    fft_sched_lds<N, RPW, NT, -1, false, LY, SCH>(xch, twc, ld2, st2);
+    stage_sync<true, NT>();
+    // whole rows, padding columns hx..HP-1 as zeros (every 128-byte line written in full)
 #pragma unroll
-    for (int i = 0; i < (RPW * HP + 63) / 64; i++) {
-      const int e = lane + i * 64;
+    for (int i = 0; i < (RPW * HP + NT - 1) / NT; i++) {
+      const int e = tid + i * NT;
       if (e >= RPW * HP) continue;
       const int t = e / HP, k = e - t * HP;
       const size_t base = (size_t)(row0 + t) * HP;
@@ -793,11 +812,23 @@ __global__ __launch_bounds__(64) void kf_xnl_dma(double2 *__restrict__ X2, const
         X2[2 * cs + base + k] = o0;
       }
     }
-    wave_lds_sync();   // the next line's first stage overwrites xch
+    stage_sync<true, NT>();   // the next line's first stage overwrites xch
   }
   }
+  // one (max |omega|^2, max |u|^2) pair per workgroup, pmax[2 blockIdx.x]: the buffer holds 2 N^2 doubles and the
+  // grid is at most N^2 / RPW workgroups (the maxima are exact, so the combining order does not matter)
   om2 = wave_max(om2); u2 = wave_max(u2);
-  if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
+  if constexpr (WPR > 1) {
+    __shared__ double red[2 * WPR];
+    if (lane == 0) { red[2 * wave] = om2; red[2 * wave + 1] = u2; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < WPR; w++) { om2 = fmax(om2, red[2 * w]); u2 = fmax(u2, red[2 * w + 1]); }
+      pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2;
+    }
+  } else {
+    if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
+  }
 }
 
 // Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred

@@ -32,6 +32,8 @@ static TuneCfg tune_default(int N) {
   TuneCfg c;
   if (N == 256 || N == 512) {
     c.xpass = NSL_XP_STAGED; c.xwaves = N == 512 ? 32 : 16; c.zfused = 1;
+  } else if (N == 1024) {
+    c.xpass = NSL_XP_STAGED; c.xwaves = 16;
   } else if (N <= 384) {
     c.xpass = NSL_XP_PAIR; c.lp = fast_LP(N);
   } else {
