@@ -248,14 +248,17 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
 template <int N> struct ZuSched;
 template <> struct ZuSched<512> { using L = RList<8, 8, 8>; };
 template <> struct ZuSched<256> { using L = RList<8, 4, 8>; };
+template <> struct ZuSched<1024> { using L = RList<16, 8, 8>; };
 #ifndef NSL_ZU_T
 #define NSL_ZU_T 8
 #endif
 #ifndef NSL_ZU_GROUP
 #define NSL_ZU_GROUP 1
 #endif
+// Lines per workgroup: 8 (whole 128-byte rows) up to 512^3; 4 at 1024^3, where 8 lines would need 1024 threads
+// with at most 128 registers each.
 template <int N> struct ZuCfg {
-  static constexpr int T = NSL_ZU_T, R = 8, NB = N / R, NT = NB * T;
+  static constexpr int T = N > 512 ? 4 : NSL_ZU_T, R = 8, NB = N / R, NT = NB * T;
   static constexpr int MINW = NT >= 512 ? 1 : 512 / NT;   // at most 256 registers (no accumulation registers)
 };
 // The last-stage slots j (elements j*N/8 .. j*N/8 + N/8 - 1) that hold a retained kz, in order.
@@ -698,9 +701,16 @@ template <> struct XnlWave<1024> { using L = RList<8, 16, 8>; };
 #ifndef NSL_XNL_BIG_MINW
 #define NSL_XNL_BIG_MINW 1
 #endif
+#ifndef NSL_XNL_BIG_TWC
+#define NSL_XNL_BIG_TWC 1
+#endif
 template <int N> struct XnlStaged {
   static constexpr int R = 8, NB = N / R, NT = NB > 64 ? NB : 64, RPW = NT / NB, WPR = NT / 64;
   static constexpr int MINW = NT > 64 ? NSL_XNL_BIG_MINW : 1;   // two-wave rows: at most 512 / MINW registers
+  // twiddles held in registers for all five transforms (TwCache). At 1024^3 reading them from the table per stage
+  // instead (NSL_XNL_BIG_TWC 0) saves 29 registers but still needs 300, and capping at 256 spills (22.9 ms against
+  // 18.3 ms)
+  static constexpr bool TWC = NT == 64 || NSL_XNL_BIG_TWC;
 };
 template <int N>
 __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
@@ -736,8 +746,10 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
     }
   };
   stage_line(0, blockIdx.x * RPW);
-  TwCache<N, RPW, NT, LY, SCH> twc;
-  twc.load(tw);
+  using TWC = typename std::conditional<C::TWC, TwCache<N, RPW, NT, LY, SCH>, const double2 *>::type;
+  TWC twc;
+  if constexpr (C::TWC) twc.load(tw);
+  else twc = tw;
   int t0, bb0;
   LY::template split<NB>(tid, t0, bb0);
   double om2 = 0.0, u2 = 0.0;

@@ -67,3 +67,17 @@ def test_pool_uses_the_override_and_matches_a_single_solve():
     c = s.state()
     s.close()
     assert float(np.max(np.abs(a - c)) / np.max(np.abs(c))) < 1e-13
+
+
+def test_1024_forms_match_the_tuned_form():
+    """1024^3 (88.6 GB per solve): the row form and the fused z-forward against the table's staged form."""
+    N = 1024
+    ref_cfg = capi.tune_get(N)
+    ref, Eref = run3(N, ref_cfg)
+    for cfg in ((2, 0, 0, 0), (4, 0, 16, 1)):
+        if cfg == ref_cfg:
+            continue
+        st, E = run3(N, cfg)
+        rel = float(np.max(np.abs(st - ref)) / np.max(np.abs(ref)))
+        assert rel < 1e-13, (cfg, rel)
+        assert np.max(np.abs(E - Eref) / Eref) < 1e-13, cfg
