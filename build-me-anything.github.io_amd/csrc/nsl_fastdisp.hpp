@@ -41,7 +41,6 @@ static bool fast_setup(Xform &x) {
 #undef NSL_CASE
     x.fast = true;
     x.fT = fast_T(x.g.N);
-    x.fLP = fast_LP(x.g.N);
     x.tune = tune_for(x.g.N);
     return true;
     default:

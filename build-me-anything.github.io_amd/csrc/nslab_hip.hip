@@ -148,7 +148,7 @@ struct Xform {
   hipStream_t st = 0;
   // size-specialised engine (nsl_fastk.hpp) when N has a compiled schedule
   bool fast = false;
-  int fT = 0, fLP = 0;
+  int fT = 0;
   TuneCfg tune;             // fixed when the transform set is made: every solve of this set runs the same kernels
   double2 *twN = nullptr;   // (cos, sin)(2 pi k / N), k < N
 };
