@@ -697,6 +697,8 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
 // Same arithmetic per line as kf_xnl_wave.
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
+// 1024^3: 8x16x8, two LDS exchanges per line. It needs 329 registers (one wave per SIMD); 2x8x8x8 and 4x4x8x8 need
+// as many (340, 328) and add an exchange.
 template <> struct XnlWave<1024> { using L = RList<8, 16, 8>; };
 #ifndef NSL_XNL_BIG_MINW
 #define NSL_XNL_BIG_MINW 1
@@ -750,8 +752,13 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
   TWC twc;
   if constexpr (C::TWC) twc.load(tw);
   else twc = tw;
-  int t0, bb0;
-  LY::template split<NB>(tid, t0, bb0);
+  // first stage: radix R0, K0 butterflies per thread (slot (k, j) is element bb + j*NB0 of line t)
+  using F0 = FirstR<SCH>;
+  constexpr int R0 = F0::R, NB0 = N / R0, K0 = NB0 * RPW / NT;
+  static_assert(K0 * NT == NB0 * RPW, "the first stage covers the workgroup");
+  int t0[K0], bb0[K0];
+#pragma unroll
+  for (int k = 0; k < K0; k++) LY::template split<NB0>(tid + k * NT, t0[k], bb0[k]);
   double om2 = 0.0, u2 = 0.0;
   for (int row0 = blockIdx.x * RPW; row0 < N * N; row0 += gridDim.x * RPW) {
   const int next = row0 + gridDim.x * RPW;
@@ -764,22 +771,28 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
     __builtin_amdgcn_s_waitcnt(0x0F70);
     if constexpr (WPR > 1) __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    double2 v[R];
+    double2 v[K0][R0];
 #pragma unroll
-    for (int j = 0; j < R; j++) {
-      v[j] = make_double2(0.0, 0.0);
-      if (j * NB >= hx && j * NB + NB - 1 <= N - hx) continue;   // zero band
-      const int n = bb0 + j * NB;
-      const bool lo = n < hx, hi = n > N - hx;
-      const int m = t0 * HP + (lo ? n : (hi ? N - n : 0));
-      const double2 A = stg[m], B = stg[FB + m];
-      const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
-      v[j] = lo || hi ? d : make_double2(0.0, 0.0);
-    }
+    for (int k = 0; k < K0; k++)
+#pragma unroll
+      for (int j = 0; j < R0; j++) {
+        v[k][j] = make_double2(0.0, 0.0);
+        // elements of slot (k, j): bb0 + j*NB0 with bb0 in [k*NT, k*NT + NT) for one row per workgroup, in
+        // [0, NB0) otherwise; a slot wholly in the zero band reads nothing
+        constexpr int span = RPW == 1 ? NT : NB0;
+        const int lo_n = j * NB0 + (RPW == 1 ? k * NT : 0);
+        if (lo_n >= hx && lo_n + span - 1 <= N - hx) continue;
+        const int n = bb0[k] + j * NB0;
+        const bool lo = n < hx, hi = n > N - hx;
+        const int m = t0[k] * HP + (lo ? n : (hi ? N - n : 0));
+        const double2 A = stg[m], B = stg[FB + m];
+        const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
+        v[k][j] = lo || hi ? d : make_double2(0.0, 0.0);
+      }
     stage_sync<true, NT>();       // every staging read is complete before the next line overwrites stg
     if (l < 2) stage_line(l + 1, row0);
     else if (next < N * N) stage_line(0, next);
-    auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
+    auto ldr = [&](int, int, int k, int j) -> double2 { return v[k][j]; };
     auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
     fft_sched_lds<N, RPW, NT, 1, false, LY, SCH>(xch, twc, ldr, keep);
   }
