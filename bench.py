@@ -243,6 +243,32 @@ def hbm_bytes_per_step(N):
     return sum(n * d[k]["bytes_per_launch"] for k, n in per_step.items())
 
 
+def fp64_vector(r):
+    """FP64 vector work of a `measure` result: flop per launch of each kernel class from the committed rocprofv3
+    SQ-counter summary (64 x (ADD + MUL + 2 FMA + TRANS) FP64 VALU instructions per dispatch), over this run's mean
+    launch times (HIP events), and the whole step's flop x steps/s, against the FP64 vector peak. The summary is of
+    the fused z-forward form; None if this run used another form or no summary exists for N."""
+    p = os.path.join(ROOT, "profiles", "pmc_valu.json")
+    if not os.path.exists(p) or not r["zfwd_fused"]:
+        return None
+    try:
+        d = json.load(open(p))
+    except (ValueError, OSError):
+        return None
+    c = d.get(str(r["N"]), {})
+    per_step = {"k_zinv": 4, "k_yinv": 4, "k_xnl": 4, "k_yfwd": 4, "k_update<1>": 1, "k_update<2|3>": 2,
+                "k_update<4>": 1}
+    if not all(k in c and k in r["kernels"] for k in per_step):
+        return None
+    peak = d["peak_tflops"]
+    flop = sum(n * c[k]["flop_per_launch"] for k, n in per_step.items())
+    tf = flop * r["steps_per_s"] / 1e12
+    ker = {k: {"tflops": round(c[k]["flop_per_launch"] / (r["kernels"][k]["avg_us"] * 1e-6) / 1e12, 2),
+               "valu_busy_pct": c[k]["valu_busy_pct"]} for k in per_step}
+    return {"flop_per_step": flop, "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+            "kernels": ker}
+
+
 def traffic_from_profile(kernel, N):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if one exists."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -319,6 +345,7 @@ def main():
                     for n, v in main_r["kernels"].items()},
         "cpu_baseline": cpu,
     }
+    out["fp64_vector"] = fp64_vector(main_r)
     hb = hbm_bytes_per_step(a.N)
     if hb:
         out["hbm_bytes_per_step_measured"] = hb
@@ -331,6 +358,7 @@ def main():
                        "hbm_bytes_per_step_measured": hb512,
                        "hbm_bytes_per_step_vs_b_alg": hb512 / b_alg_step(512) if hb512 else None,
                        "ms_per_step": extra["ms_per_step"], "step_roofline_frac": extra["step_roofline_frac"],
+                       "fp64_vector": fp64_vector(extra),
                        "dominant": extra["dominant"],
                        "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}

@@ -193,15 +193,16 @@ __device__ __forceinline__ void block_max2_store(double a, double b, double *dst
 
 // Fixed-order tree sum of NV values per thread across the workgroup; thread 0 gets the totals in out[].
 // The order depends only on the launch shape, so results are reproducible run to run and batch to batch.
+// A workgroup that is not a power of two (48, 96, 192 threads) sums as if padded with zeros to the next one.
+__host__ __device__ constexpr int pow2_ceil(int n) { return n <= 1 ? 1 : 2 * pow2_ceil((n + 1) / 2); }
 template <int NV, int NT = NSL_NT>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch /* NT*NV */, double *out) {
-  static_assert((NT & (NT - 1)) == 0, "power-of-two workgroup");
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] = v[i];
   __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
+  for (int w = pow2_ceil(NT) / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w && (int)threadIdx.x + w < NT) {
 #pragma unroll
       for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] += scratch[i * NT + threadIdx.x + w];
     }

@@ -113,7 +113,10 @@ static constexpr bool xp_has(int N, int form) {
        : form == NSL_XP_WAVE ? (N == 256 || N == 512)
        : form == NSL_XP_STAGED ? (N == 256 || N == 512 || N == 1024) : false;
 }
-static constexpr bool zu_has(int N) { return N == 256 || N == 512 || N == 1024; }
+static constexpr bool zu_has(int N) {
+  return N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 ||
+         N == 1024;
+}
 
 template <int N, int LP>
 static int fast_xnl_pair(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
@@ -220,7 +223,7 @@ static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl
 static bool fast_has_zu(const Xform &x) { return x.fast && zu_has(x.g.N) && x.tune.zfused; }
 // Workgroups (and sum partials) of the fused pass.
 static int fast_zu_blocks(const Geo &g) {
-  const int T = g.N > 512 ? 4 : NSL_ZU_T;   // fast::ZuCfg<N>::T
+  const int T = fast::zu_lines(g.N);   // fast::ZuCfg<N>::T
   return ((g.hx + T - 1) / T) * g.by;
 }
 template <int N, int STAGE>
@@ -248,7 +251,14 @@ static int fast_zu_n(const Xform &x, int stage, const double2 *X4, const double2
 static int fast_dispatch_zu(const Xform &x, int stage, const double2 *X4, const double2 *U, double2 *S, double2 *ACC,
                             double2 *TS, double *partials, Ctl *ctl, const Batch &bt) {
   switch (x.g.N) {
+    case 32: return fast_zu_n<32>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 48: return fast_zu_n<48>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 64: return fast_zu_n<64>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 96: return fast_zu_n<96>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 128: return fast_zu_n<128>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 192: return fast_zu_n<192>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
     case 256: return fast_zu_n<256>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
+    case 384: return fast_zu_n<384>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
     case 512: return fast_zu_n<512>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
     case 1024: return fast_zu_n<1024>(x, stage, X4, U, S, ACC, TS, partials, ctl, bt);
     default: return fail(NSL_ERR_STATE, "no fused z-forward for this N");

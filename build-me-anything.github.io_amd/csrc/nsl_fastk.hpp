@@ -246,8 +246,15 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
 // per line: slot j of thread x is element n = x/T + j*N/8 of line x%T, for all three components at once.
 // Sums: one partial per workgroup (fixed-order tree), grid ntile*by.
 template <int N> struct ZuSched;
-template <> struct ZuSched<512> { using L = RList<8, 8, 8>; };
+template <> struct ZuSched<32> { using L = RList<4, 8>; };
+template <> struct ZuSched<48> { using L = RList<2, 3, 8>; };
+template <> struct ZuSched<64> { using L = RList<8, 8>; };
+template <> struct ZuSched<96> { using L = RList<4, 3, 8>; };
+template <> struct ZuSched<128> { using L = RList<16, 8>; };
+template <> struct ZuSched<192> { using L = RList<8, 3, 8>; };
 template <> struct ZuSched<256> { using L = RList<8, 4, 8>; };
+template <> struct ZuSched<384> { using L = RList<16, 3, 8>; };
+template <> struct ZuSched<512> { using L = RList<8, 8, 8>; };
 template <> struct ZuSched<1024> { using L = RList<16, 8, 8>; };
 #ifndef NSL_ZU_T
 #define NSL_ZU_T 8
@@ -255,11 +262,14 @@ template <> struct ZuSched<1024> { using L = RList<16, 8, 8>; };
 #ifndef NSL_ZU_GROUP
 #define NSL_ZU_GROUP 1
 #endif
-// Lines per workgroup: 8 (whole 128-byte rows) up to 512^3; 4 at 1024^3, where 8 lines would need 1024 threads
-// with at most 128 registers each.
+// Lines per workgroup: 8 (whole 128-byte rows); 16 at 32^3 (one wave: 4 threads per line, and the row pitch is 16);
+// 4 at 1024^3, where 8 lines would need 1024 threads with at most 128 registers each.
+__host__ __device__ constexpr int zu_lines(int N) { return N > 512 ? 4 : N == 32 ? 16 : NSL_ZU_T; }
 template <int N> struct ZuCfg {
-  static constexpr int T = N > 512 ? 4 : NSL_ZU_T, R = 8, NB = N / R, NT = NB * T;
-  static constexpr int MINW = NT >= 512 ? 1 : 512 / NT;   // at most 256 registers (no accumulation registers)
+  static constexpr int T = zu_lines(N), R = 8, NB = N / R, NT = NB * T;
+  // at most 256 registers (512 from 512 threads up: one workgroup per CU, no accumulation registers needed);
+  // a cap of 168 or 128 spills 58-146 registers
+  static constexpr int MINW = NT >= 512 ? 1 : 2;
 };
 // The last-stage slots j (elements j*N/8 .. j*N/8 + N/8 - 1) that hold a retained kz, in order.
 template <int N> struct ZuSlots {

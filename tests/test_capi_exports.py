@@ -69,7 +69,7 @@ def test_tune_table_and_overrides_without_gpu():
             assert cur == table[N], N
     assert len(capi.tune_candidates(256)) == 18 and len(capi.tune_candidates(1024)) == 12
     # forms not compiled for a size are refused
-    for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (128, (1, 2, 0, 1)), (512, (1, 1, 0, 0))):
+    for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (768, (2, 0, 0, 1)), (512, (1, 1, 0, 0))):
         try:
             capi.tune_set(N, bad)
             raise AssertionError("expected an error for %s at %d" % (bad, N))

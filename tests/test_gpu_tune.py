@@ -2,7 +2,8 @@
 
 For N = 64 (row-pair tiles 1, 2, 4, 8), 256 (row pairs, wave, staged wave with and without persistent waves, fused
 and unfused z-forward) and 512 (rows, wave, staged, fused or not), a random-IC solve (CFL 0.5) run for 3 steps under
-each form matches the same solve under the table's form to 1e-13 relative in the state and the series. The forms
+each form matches the same solve under the table's form to 1e-13 relative in the state and the series; at the
+other sizes with a fused z-forward, the fused and two-kernel forms do. The forms
 differ only in the radix factorisation and tiling of the same transforms, so the expected difference is roundoff
 (a few 1e-16). Also: a pool created under an override uses it for every slot, and the slot stays bit-identical to a
 single solve created under the same override."""
@@ -41,6 +42,20 @@ def test_every_form_matches_the_tuned_form(N):
         rel = float(np.max(np.abs(st - ref)) / np.max(np.abs(ref)))
         assert rel < 1e-13, (N, cfg, rel)
         assert np.max(np.abs(E - Eref) / Eref) < 1e-13, (N, cfg)
+
+
+@pytest.mark.parametrize("N", [32, 48, 96, 128, 192, 384])
+def test_fused_and_unfused_z_forward_agree(N):
+    """The z-forward fused with the update (radix schedules ending in 8: 4*8, 2*3*8, 4*3*8, 16*8, 8*3*8, 16*3*8) and
+    the two-kernel form, each against the table's form."""
+    ref_cfg = capi.tune_get(N)
+    flip = ref_cfg[:3] + (1 - ref_cfg[3],)
+    assert flip in capi.tune_candidates(N), (N, flip)
+    ref, Eref = run3(N, ref_cfg)
+    st, E = run3(N, flip)
+    rel = float(np.max(np.abs(st - ref)) / np.max(np.abs(ref)))
+    assert rel < 1e-13, (N, flip, rel)
+    assert np.max(np.abs(E - Eref) / Eref) < 1e-13, (N, flip)
 
 
 def test_pool_uses_the_override_and_matches_a_single_solve():

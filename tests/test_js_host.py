@@ -90,5 +90,5 @@ def test_autotuner_runs_and_checks_every_form(tmp_path):
     assert [x["N"] for x in res] == [32, 256]
     for x in res:
         forms = [c["form"] for c in x["candidates"]]
-        assert x["pick"] in forms and len(forms) == (4 if x["N"] == 32 else 18)
+        assert x["pick"] in forms and len(forms) == (8 if x["N"] == 32 else 18)
         assert all(c["rel_diff_E"] < 1e-11 and c["steps_per_s"] > 0 for c in x["candidates"])
