@@ -610,6 +610,8 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
 // only the exchanges inside each line's transform, one line of each row at a time, and the half-spectra are read
 // straight from HBM into the first stage (Hermitian mirror of modes N-k in place).
 template <int N> struct XnlWave;
+template <> struct XnlWave<64> { using L = RList<8, 8>; };
+template <> struct XnlWave<128> { using L = RList<8, 2, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
 template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
 template <int N>
@@ -618,6 +620,7 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
   constexpr int R = 8, NB = N / R, RPW = 64 / NB;   // butterflies per line and stage ends, rows per wave
   static_assert(NB * RPW == 64, "first and last stage must cover a wave");
   using SCH = typename XnlWave<N>::L;
+  static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in radix 8");
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
   extern __shared__ double2 lds[];
@@ -733,6 +736,9 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
   constexpr int R = C::R, NB = C::NB, NT = C::NT, RPW = C::RPW, WPR = C::WPR;
   static_assert(NB * RPW == NT, "first and last stage: one butterfly per thread");
   using SCH = typename XnlWave<N>::L;
+  // the forward transforms start from the inverse's last-stage registers: slot j of the radix-R last stage holds
+  // element bb + j*N/R, which is what a radix-R first stage reads, so the schedule starts and ends in radix R
+  static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in radix 8");
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
   constexpr size_t cs = (size_t)N * N * HP;

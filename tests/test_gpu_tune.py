@@ -1,6 +1,6 @@
 """Every compiled kernel form of a size gives the tuned form's step: the autotuner may pick any of them.
 
-For N = 64 (row-pair tiles 1, 2, 4, 8), 256 (row pairs, wave, staged wave with and without persistent waves, fused
+For N = 64 and 128 (row-pair tiles 1, 2, 4, 8, staged wave with and without persistent waves), 256 (row pairs, wave, staged wave with and without persistent waves, fused
 and unfused z-forward) and 512 (rows, wave, staged, fused or not), a random-IC solve (CFL 0.5) run for 3 steps under
 each form matches the same solve under the table's form to 1e-13 relative in the state and the series; at the
 other sizes with a fused z-forward, the fused and two-kernel forms do. The forms
@@ -28,7 +28,7 @@ def run3(N, cfg):
         capi.tune_set(N)
 
 
-@pytest.mark.parametrize("N", [64, 256, 512])
+@pytest.mark.parametrize("N", [64, 128, 256, 512])
 def test_every_form_matches_the_tuned_form(N):
     ref_cfg = capi.tune_get(N)
     ref, Eref = run3(N, ref_cfg)
