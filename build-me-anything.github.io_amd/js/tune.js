@@ -4,12 +4,14 @@
  * persistent waves of the staged x pass, z-forward fused with the update or not; nsl_tune_candidates) on this
  * device, checks that each gives the same step as the current form, and writes the table the library compiles in
  * (csrc/nsl_tune_table.inc) plus a JSON report. A form replaces the current one only when it is faster by more
- * than the noise margin, so re-running on the same device keeps the table stable.
+ * than the noise margin, so re-running on the same device keeps the table stable. Up to --pool-max (256) each form
+ * is also timed in a pool of slots (the batched path: one launch per pass covers every slot), and a form's score is
+ * the geometric mean of its single-solve and pool speeds relative to the current form.
  *
  * The table is a build input: solvers then pick their forms by N alone (a solve gives the same bits alone and in
  * any pool). NSLAB_TUNE=<file> ("N xpass lp xwaves zfused" per line) overrides it at run time without a rebuild.
  *
- *   node js/tune.js [--sizes 32,48,64,96,128,192,256,384,512] [--budget 0.6] [--margin 0.02]
+ *   node js/tune.js [--sizes 32,48,64,96,128,192,256,384,512] [--budget 0.6] [--margin 0.02] [--pool-max 256]
  *                   [--out csrc/nsl_tune_table.inc] [--report tune_report.json] [--dry]
  */
 const fs = require('fs');
@@ -20,7 +22,7 @@ const FORM = { 1: 'row pairs', 2: 'rows', 3: 'wave', 4: 'staged wave' };
 const REC = 21;   // doubles per step record of addon.run: t dt step ebal zbal last[8] d0[8]
 
 function args() {
-  const a = { sizes: [32, 48, 64, 96, 128, 192, 256, 384, 512], budget: 0.6, margin: 0.02, dry: false,
+  const a = { sizes: [32, 48, 64, 96, 128, 192, 256, 384, 512], budget: 0.6, margin: 0.02, poolMax: 256, dry: false,
     out: path.join(__dirname, '..', 'csrc', 'nsl_tune_table.inc'), report: null };
   const v = process.argv.slice(2);
   for (let i = 0; i < v.length; i++) {
@@ -30,6 +32,7 @@ function args() {
     else if (k === '--margin') a.margin = +v[++i];
     else if (k === '--out') a.out = v[++i];
     else if (k === '--report') a.report = v[++i];
+    else if (k === '--pool-max') a.poolMax = +v[++i];
     else if (k === '--dry') a.dry = true;
     else throw new Error('unknown argument ' + k);
   }
@@ -66,6 +69,44 @@ function timeForm(N, cfg, budget) {
   }
 }
 
+// Slots of the pool a form is timed in: about 16 GB of slots, 2 to 32.
+const poolSlots = (N) => Math.max(2, Math.min(32, Math.floor(16e9 / addon.poolSlotBytes(N))));
+
+// Solve-steps/s of the forms `cfg` at N in a pool of `slots` slots, the same solve as timeForm in every slot: 3 steps
+// untimed (the step graph is captured on the first launch), then about `budget` seconds of 64-step chunks. The pool's
+// device memory is freed before returning.
+function timePool(N, cfg, slots, budget) {
+  addon.tuneSet(N, cfg[0], cfg[1], cfg[2], cfg[3]);
+  const p = addon.poolCreate(N, slots);
+  try {
+    const ids = [];
+    for (let i = 0; i < slots; i++) ids.push(addon.poolAdmit(p, N, 0, 1 / 1600, 0.4, 1e-3, 0, 0, 0, 0)[0]);
+    const run = (n) => {
+      for (const s of ids) addon.poolSetTarget(p, s, n, 0, 0);
+      for (let left = n; left > 0; left -= 64) {
+        const k = Math.min(64, left);
+        addon.poolLaunch(p, k);
+        addon.poolCollect(p, k);
+      }
+    };
+    run(3);
+    let n = 4, el = 0;
+    for (;;) {
+      const t0 = now();
+      run(n);
+      el = now() - t0;
+      if (el > 0.25 * budget || n >= 1024) break;
+      n *= 2;
+    }
+    const reps = Math.max(1, Math.round(budget / Math.max(el, 1e-6)));
+    const t0 = now();
+    run(n * reps);
+    return (slots * n * reps) / (now() - t0);
+  } finally {
+    addon.poolDestroy(p);
+  }
+}
+
 function tuneN(N, a) {
   const cur = addon.tuneGet(N);
   const cands = addon.tuneCandidates(N);
@@ -77,22 +118,31 @@ function tuneN(N, a) {
     const m = timeForm(N, c, a.budget);
     if (ref === null) ref = m;
     const same = Math.abs(m.E - ref.E) / Math.abs(ref.E);
-    rows.push({ form: c, what: describe(c), steps_per_s: m.sps, rel_diff_E: same });
+    const row = { form: c, what: describe(c), steps_per_s: m.sps, rel_diff_E: same };
+    if (N <= a.poolMax) {
+      row.pool_slots = poolSlots(N);
+      row.pool_solve_steps_per_s = timePool(N, c, row.pool_slots, a.budget);
+    }
+    rows.push(row);
     if (!(same < 1e-11)) throw new Error(`N=${N} form ${describe(c)}: energy after 3 steps differs by ${same}`);
+  }
+  for (const r of rows) {
+    r.score = r.steps_per_s / rows[0].steps_per_s;
+    if (r.pool_solve_steps_per_s) r.score = Math.sqrt(r.score * r.pool_solve_steps_per_s / rows[0].pool_solve_steps_per_s);
   }
   addon.tuneSet(N);   // drop the override
   const pick = choose(rows, a.margin);
   return { N, current: describe(cur), pick: pick.form, pick_what: pick.what, kept_current: pick === rows[0],
-    gain: pick.steps_per_s / rows[0].steps_per_s, candidates: rows };
+    gain: pick.score, candidates: rows };
 }
 
-// The form to keep, from timed rows (the current form first). Forms within the noise margin of the fastest are a
-// tie. A tie keeps the current form, except that the fused z-forward of the same x-pass form wins it: it moves
+// The form to keep, from timed rows (the current form first; `score` is the speed relative to it). Forms within the
+// noise margin of the best score are a tie. A tie keeps the current form, except that the fused z-forward of the same x-pass form wins it: it moves
 // 6 D complex values (D retained modes) fewer per rhs, which counts most in a pool, where every pass is
 // bandwidth-bound. Without the current form among the tied ones, the fastest is taken.
 function choose(rows, margin) {
-  const best = rows.reduce((b, r) => (r.steps_per_s > b.steps_per_s ? r : b), rows[0]);
-  const tied = (r) => r.steps_per_s * (1 + margin) >= best.steps_per_s;
+  const best = rows.reduce((b, r) => (r.score > b.score ? r : b), rows[0]);
+  const tied = (r) => r.score * (1 + margin) >= best.score;
   const cur = rows[0];
   if (!tied(cur)) return best;
   if (cur.form[3] === 0) {
@@ -115,7 +165,8 @@ if (require.main === module) {
     const r = tuneN(N, a);
     results.push(r);
     process.stdout.write(`N=${N}: ${r.pick_what}${r.kept_current ? ' (current kept)' : ''}, x${r.gain.toFixed(3)}; ` +
-      r.candidates.map((c) => `${c.what} ${c.steps_per_s.toFixed(1)}`).join(' | ') + '\n');
+      r.candidates.map((c) => `${c.what} ${c.steps_per_s.toFixed(1)}` +
+        (c.pool_solve_steps_per_s ? `/${c.pool_solve_steps_per_s.toFixed(0)} in ${c.pool_slots}` : '')).join(' | ') + '\n');
   }
   if (!a.dry) writeTable(a.out, results);
   if (a.report) fs.writeFileSync(a.report, JSON.stringify({ device: addon.version(), results }, null, 1));

@@ -411,15 +411,22 @@ static napi_value js_profile_read(napi_env env, napi_callback_info info) {
 }
 
 /* ------------------------------------------------------------------------------------------------ pools */
+/* A pool handle: freed by poolDestroy or, failing that, when the handle is collected. */
+typedef struct {
+  nsl_pool *p;
+} pool_ref;
+
 static void finalize_pool(napi_env env, void *data, void *hint) {
   (void)env; (void)hint;
-  nsl_pool_destroy((nsl_pool *)data);
+  pool_ref *r = (pool_ref *)data;
+  if (r->p) nsl_pool_destroy(r->p);
+  free(r);
 }
 
 static nsl_pool *get_pool(napi_env env, napi_value v) {
   void *p = NULL;
-  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
-  return (nsl_pool *)p;
+  if (napi_get_value_external(env, v, &p) != napi_ok || p == NULL) return NULL;
+  return ((pool_ref *)p)->p;
 }
 
 #define POOL(idx)                                                           \
@@ -446,9 +453,31 @@ static napi_value js_pool_create(napi_env env, napi_callback_info info) {
   nsl_pool *p = NULL;
   int rc = nsl_pool_create((int32_t)N, (int32_t)slots, &p);
   if (rc < 0) return throw_status(env, rc);
+  pool_ref *r = (pool_ref *)malloc(sizeof(pool_ref));
+  if (!r) {
+    nsl_pool_destroy(p);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  r->p = p;
   napi_value ext;
-  CALL(env, napi_create_external(env, p, finalize_pool, NULL, &ext));
+  CALL(env, napi_create_external(env, r, finalize_pool, NULL, &ext));
   return ext;
+}
+
+/* poolDestroy(p): frees the pool's device memory now. The pool's slot handles must not be used afterwards; the pool
+ * handle then throws "not a pool handle". */
+static napi_value js_pool_destroy(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  void *d = NULL;
+  if (napi_get_value_external(env, argv[0], &d) != napi_ok || d == NULL) {
+    napi_throw_type_error(env, NULL, "not a pool handle");
+    return NULL;
+  }
+  pool_ref *r = (pool_ref *)d;
+  if (r->p) nsl_pool_destroy(r->p);
+  r->p = NULL;
+  return NULL;
 }
 
 /* poolInfo(p) -> [N, slots, used, graphed, bytes] */
@@ -667,6 +696,7 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"version", js_version}, {"poolSlotBytes", js_pool_slot_bytes}, {"poolCreate", js_pool_create},
     {"poolInfo", js_pool_info}, {"poolAdmit", js_pool_admit}, {"poolSetTarget", js_pool_set_target},
     {"poolLaunch", js_pool_launch}, {"poolCollect", js_pool_collect}, {"poolRelease", js_pool_release},
+    {"poolDestroy", js_pool_destroy},
     {"deviceMemory", js_device_memory}, {"poolReady", js_pool_ready},
     {"tuneGet", js_tune_get}, {"tuneSet", js_tune_set}, {"tuneCandidates", js_tune_candidates},
   };

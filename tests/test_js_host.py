@@ -77,8 +77,9 @@ def test_parity_gate_grades_series():
 @needs_node
 @pytest.mark.gpu
 def test_autotuner_runs_and_checks_every_form(tmp_path):
-    """js/tune.js times every compiled form at 32^3 and 256^3 (briefly), checks each against the current form's
-    energy after 3 steps, and reports a pick among the candidates; --dry leaves the compiled-in table alone."""
+    """js/tune.js times every compiled form at 32^3 and 256^3 (briefly), alone and in a pool, checks each against the
+    current form's energy after 3 steps, and reports a pick among the candidates; --dry leaves the compiled-in table
+    alone."""
     import json
     script = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "tune.js")
     rep = str(tmp_path / "tune.json")
@@ -92,3 +93,7 @@ def test_autotuner_runs_and_checks_every_form(tmp_path):
         forms = [c["form"] for c in x["candidates"]]
         assert x["pick"] in forms and len(forms) == (8 if x["N"] == 32 else 18)
         assert all(c["rel_diff_E"] < 1e-11 and c["steps_per_s"] > 0 for c in x["candidates"])
+        # both sizes are timed in pools too (up to --pool-max 256), and the pick has the best score within 2 %
+        assert all(c["pool_solve_steps_per_s"] > 0 and c["pool_slots"] >= 2 for c in x["candidates"])
+        best = max(c["score"] for c in x["candidates"])
+        assert x["gain"] * 1.02 >= best
