@@ -111,7 +111,9 @@ static constexpr bool xp_has(int N, int form) {
   return form == NSL_XP_PAIR ? N <= 384
        : form == NSL_XP_ROWS ? N >= 384
        : form == NSL_XP_WAVE ? (N == 256 || N == 512)
-       : form == NSL_XP_STAGED ? (N == 64 || N == 128 || N == 256 || N == 512 || N == 1024) : false;
+       : form == NSL_XP_STAGED ? (N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 ||
+                                  N == 384 || N == 512 || N == 1024)
+       : false;
 }
 static constexpr bool zu_has(int N) {
   return N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 ||

@@ -610,8 +610,13 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
 // only the exchanges inside each line's transform, one line of each row at a time, and the half-spectra are read
 // straight from HBM into the first stage (Hermitian mirror of modes N-k in place).
 template <int N> struct XnlWave;
+template <> struct XnlWave<32> { using L = RList<4, 2, 4>; };
+template <> struct XnlWave<48> { using L = RList<4, 3, 4>; };
 template <> struct XnlWave<64> { using L = RList<8, 8>; };
+template <> struct XnlWave<96> { using L = RList<4, 3, 2, 4>; };
 template <> struct XnlWave<128> { using L = RList<8, 2, 8>; };
+template <> struct XnlWave<192> { using L = RList<8, 3, 8>; };
+template <> struct XnlWave<384> { using L = RList<8, 3, 2, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
 template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
 template <int N>
@@ -719,9 +724,16 @@ template <> struct XnlWave<1024> { using L = RList<8, 16, 8>; };
 #ifndef NSL_XNL_BIG_TWC
 #define NSL_XNL_BIG_TWC 1
 #endif
+__host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+// The staged form's shape: stage-end radix R (8, or 4 where N has no 8.....8 factorisation: 32, 48, 96), NB = N/R
+// butterflies per line, and the smallest workgroup of whole waves that holds whole rows, lcm(NB, 64) threads: one
+// wave for RPW = 64/NB rows up to 512^3, two waves per row at 1024^3, three waves for 8 or 16 rows at 48, 96, 192
+// and 384.
 template <int N> struct XnlStaged {
-  static constexpr int R = 8, NB = N / R, NT = NB > 64 ? NB : 64, RPW = NT / NB, WPR = NT / 64;
-  static constexpr int MINW = NT > 64 ? NSL_XNL_BIG_MINW : 1;   // two-wave rows: at most 512 / MINW registers
+  static constexpr int R = FirstR<typename XnlWave<N>::L>::R, NB = N / R;
+  static constexpr int NT = NB / gcd_int(NB, 64) * 64, RPW = NT / NB, WPR = NT / 64;
+  // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
+  static constexpr int MINW = NT == 128 ? NSL_XNL_BIG_MINW : NT > 64 ? 2 : 1;
   // twiddles held in registers for all five transforms (TwCache). At 1024^3 reading them from the table per stage
   // instead (NSL_XNL_BIG_TWC 0) saves 29 registers but still needs 300, and capping at 256 spills (22.9 ms against
   // 18.3 ms)
@@ -738,7 +750,7 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
   using SCH = typename XnlWave<N>::L;
   // the forward transforms start from the inverse's last-stage registers: slot j of the radix-R last stage holds
   // element bb + j*N/R, which is what a radix-R first stage reads, so the schedule starts and ends in radix R
-  static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in radix 8");
+  static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in the same radix");
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
   constexpr size_t cs = (size_t)N * N * HP;
