@@ -715,14 +715,12 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
 // Same arithmetic per line as kf_xnl_wave.
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
-// 1024^3: 8x16x8, two LDS exchanges per line. It needs 329 registers (one wave per SIMD); 2x8x8x8 and 4x4x8x8 need
-// as many (340, 328) and add an exchange.
-template <> struct XnlWave<1024> { using L = RList<8, 16, 8>; };
+// 1024^3: 8x8x2x8 with table twiddles, 248 registers (two waves per SIMD), 17.9 ms against 18.5 ms for 8x16x8 with
+// cached twiddles (329 registers, one wave per SIMD). Two waves per row; 4x8x8x4 and 4x4x4x4x4 with four waves per
+// row, and staging whole rows instead of lines, measured no faster.
+template <> struct XnlWave<1024> { using L = RList<8, 8, 2, 8>; };
 #ifndef NSL_XNL_BIG_MINW
 #define NSL_XNL_BIG_MINW 1
-#endif
-#ifndef NSL_XNL_BIG_TWC
-#define NSL_XNL_BIG_TWC 1
 #endif
 __host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
 // The staged form's shape: stage-end radix R (8, or 4 where N has no 8.....8 factorisation: 32, 48, 96), NB = N/R
@@ -734,10 +732,10 @@ template <int N> struct XnlStaged {
   static constexpr int NT = NB / gcd_int(NB, 64) * 64, RPW = NT / NB, WPR = NT / 64;
   // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
   static constexpr int MINW = NT == 128 ? NSL_XNL_BIG_MINW : NT > 64 ? 2 : 1;
-  // twiddles held in registers for all five transforms (TwCache). At 1024^3 reading them from the table per stage
-  // instead (NSL_XNL_BIG_TWC 0) saves 29 registers but still needs 300, and capping at 256 spills (22.9 ms against
-  // 18.3 ms)
-  static constexpr bool TWC = NT == 64 || NSL_XNL_BIG_TWC;
+  // one-wave groups hold the twiddles of all five transforms in registers (TwCache); multi-wave groups read them
+  // from the table at each stage: at 1024^3 (8.8.2.8) the cache is the difference between 248 and 300+ registers
+  // (two waves per SIMD or one), at 192^3 it spills 14 registers
+  static constexpr bool TWC = NT == 64;
 };
 template <int N>
 __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
