@@ -69,5 +69,22 @@ const sameSeries = (a, b) => ['t', 'E', 'Z', 'eps', 'omMax', 'uMax', 'dt', 'ebal
   check('serve: ladder verdicts present', mix.results.every(r => r.summary && ['PASS', 'WARN', 'FAIL'].includes(r.summary.verdict)));
 }
 
+// 4. poolDestroy frees a pool's device memory at once; the handle is then refused
+{
+  const addon = require(path.join(pkg, '..', 'lib', 'nslab_napi.node'));
+  const slotBytes = addon.poolSlotBytes(128);
+  const before = NS.deviceMemory().free;
+  const p = addon.poolCreate(128, 8);
+  const held = before - NS.deviceMemory().free;
+  addon.poolAdmit(p, 128, 0, 1 / 1600, 0.4, 1e-3, 0, 0, 0, 0);
+  addon.poolDestroy(p);
+  const back = NS.deviceMemory().free;
+  check('poolDestroy: the pool held its slots', held >= 8 * slotBytes * 0.99, `${held} B for 8 x ${slotBytes} B`);
+  check('poolDestroy: memory returned at once', before - back < slotBytes, `${before - back} B still held`);
+  let threw = false;
+  try { addon.poolInfo(p); } catch (e) { threw = /not a pool handle/.test(e.message); }
+  check('poolDestroy: a destroyed pool handle is refused', threw);
+}
+
 console.log(fails ? `${fails} FAILED` : 'ALL PASS');
 process.exit(fails ? 1 : 0);
