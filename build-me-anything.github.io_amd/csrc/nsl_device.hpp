@@ -201,7 +201,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch /* NT
 #pragma unroll
   for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] = v[i];
   __syncthreads();
-  for (int w = pow2_ceil(NT) / 2; w > 0; w >>= 1) {
+  constexpr int P2 = pow2_ceil(NT);   // a constant: evaluated by the compiler, not called at run time
+  for (int w = P2 / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w && (int)threadIdx.x + w < NT) {
 #pragma unroll
       for (int i = 0; i < NV; i++) scratch[i * NT + threadIdx.x] += scratch[i * NT + threadIdx.x + w];

@@ -84,11 +84,12 @@ def test_pool_uses_the_override_and_matches_a_single_solve():
 
 
 def test_1024_forms_match_the_tuned_form():
-    """1024^3 (88.6 GB per solve): the row form and the fused z-forward against the table's staged form."""
+    """1024^3 (88.6 GB per solve): the row form and the staged form with and without the fused z-forward against
+    the default form."""
     N = 1024
     ref_cfg = capi.tune_get(N)
     ref, Eref = run3(N, ref_cfg)
-    for cfg in ((2, 0, 0, 0), (4, 0, 16, 1)):
+    for cfg in ((2, 0, 0, 0), (4, 0, 16, 0), (4, 0, 16, 1)):
         if cfg == ref_cfg:
             continue
         st, E = run3(N, cfg)
