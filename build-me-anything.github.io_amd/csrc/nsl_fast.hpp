@@ -158,7 +158,7 @@ template <int N, int T> struct Interleaved {
   static __device__ __forceinline__ int at(int n, int t) { return n * T + t; }
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { t = idx % T; bb = idx / T; }
 };
-template <int N, int T> struct LineMajor {
+template <int N, int T> struct LineMajor {   // one pad slot per 8 (per 4: x pass 184 -> 212 us at 256^3; per 16: 187)
   static constexpr int STRIDE = N + N / 8, SIZE = T * STRIDE;
   static __device__ __forceinline__ int at(int n, int t) { return t * STRIDE + n + (n >> 3); }
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { bb = idx % NB; t = idx / NB; }
