@@ -615,6 +615,8 @@ template <> struct XnlWave<48> { using L = RList<4, 3, 4>; };
 template <> struct XnlWave<64> { using L = RList<8, 8>; };
 template <> struct XnlWave<96> { using L = RList<4, 3, 2, 4>; };
 template <> struct XnlWave<128> { using L = RList<8, 2, 8>; };
+// 192^3 and 384^3 with radix-4 ends (4.3.4.4, 4.4.3.4; 4.3.8.4, 4.8.3.4) need fewer registers (166-208) but add an
+// LDS exchange: 150-205 us against 140-146 us at 192^3, 1646-1652 against 1477 us at 384^3
 template <> struct XnlWave<192> { using L = RList<8, 3, 8>; };
 template <> struct XnlWave<384> { using L = RList<8, 3, 2, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
