@@ -10,10 +10,10 @@ event pair around every launch, for the per-kernel averages (the events add gaps
 Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
 its mean launch time, from HIP events recorded on the solver's stream inside the second timed region), the whole-step
 roofline fraction of the truncated-storage traffic model, a 512^3 line, the CPU baseline (the oracle, a
-single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and two lines from
-the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step) and the
-ladder request loop (Poisson arrivals, admission by HBM footprint, p50/p99 latency, ladders per hour, against
-sequential single-solve execution of the same work).
+single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and three lines
+from the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step),
+the ladder request loop (Poisson arrivals, admission by HBM footprint, p50/p99 latency, ladders per hour, against
+sequential single-solve execution of the same work), and the same loop with half the requests Reynolds sweeps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256] [--no-512] [--no-cpu] [--no-ensemble]
                   [--no-serve] [--serve-requests R]
@@ -393,6 +393,18 @@ def main():
             "sequential_sample_s": sv.get("sequential_sample_s"),
             "sequential_sample_estimate_s": sv.get("sequential_sample_estimate_s"),
             "served_equals_sequential": sv.get("served_equals_sequential")}
+        # the same loop with half the requests Reynolds sweeps (one grid of 64..192, three Re of the sqrt(2) series)
+        sm = node_leg(["serve", "--requests", str(max(4, a.serve_requests // 2)), "--rate", "2", "--seed", "2",
+                       "--sweeps", "0.5", "--seq-sample", "2"], 900)
+        out["serving_mixed"] = sm if "error" in sm else {
+            "workload": "%d requests, Poisson 2/s, half refinement ladders as above, half Reynolds sweeps (one grid "
+                        "from 64/96/128/192, three consecutive Re of 707..4000 in steps of sqrt(2)), tubes or random "
+                        "IC, 200-1000 steps, CFL 0.5" % sm["requests"],
+            "requests_per_hour": sm["requests_per_hour"], "sweeps": sm["sweeps"],
+            "rung_solves_per_hour": sm["rung_solves_per_hour"], "p50_latency_s": sm["p50_latency_s"],
+            "p99_latency_s": sm["p99_latency_s"], "wall_s": sm["wall_s"],
+            "throughput_vs_sequential": sm["throughput_vs_sequential"],
+            "served_equals_sequential": sm.get("served_equals_sequential")}
     print(json.dumps(out))
 
 

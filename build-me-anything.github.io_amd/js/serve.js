@@ -3,14 +3,16 @@
  *
  * runEnsemble(): many independent solves of uneven N stepped together, one pool per N, every pool's RK4 step a
  *   captured hipGraph, the pools on their own streams so they overlap on the device.
- * serve(): refinement-ladder requests arrive as a Poisson process; each is admitted (all rungs at once) when the
- *   pools have free slots or the HBM budget allows a new pool, runs resident on the device beside the other
- *   requests, and completes with the reference's ladder verdict (studySummary, nslab.js:652-666, over the per-level
- *   peaks that nslab-worker.js:15-19 books). Reports p50/p99 completion latency and ladders per hour.
+ * serve(): refinement-ladder and Reynolds-sweep requests arrive as a Poisson process; each is admitted (all members
+ *   at once) when the pools have free slots or the HBM budget allows a new pool, runs resident on the device beside
+ *   the other requests, and completes with its verdict: a ladder the reference's (studySummary, nslab.js:652-666,
+ *   over the per-level peaks that nslab-worker.js:15-19 books), a sweep the peaks' scaling with Re (sweepSummary).
+ *   Reports p50/p99 completion latency and requests per hour.
  *
  * CLI (one JSON object on stdout):
  *   node serve.js ensemble [--steps 300] [--seq-per-n 8]
- *   node serve.js serve [--requests 60] [--rate 2] [--seed 1] [--chunk 32] [--budget-gb 250]
+ *   node serve.js serve [--requests 60] [--rate 2] [--seed 1] [--chunk 32] [--budget-gb 250] [--sweeps 0]
+ *                       (--sweeps F: the fraction of requests that are Reynolds sweeps)
  */
 'use strict';
 const path = require('path');
@@ -126,20 +128,57 @@ function runEnsemble(opts = {}) {
 
 // ------------------------------------------------------------------------------------------------ serving
 const LADDERS = [[48, 64, 96], [64, 96, 128], [96, 128, 192], [128, 192, 256]];
+// Reynolds sweeps: one grid, three consecutive Re of the sqrt(2) series 707 .. 4000 that the reference's tube runs
+// use (research/nslab: Re 707, 1000, 1414, 2000)
+const SWEEP_GRIDS = [64, 96, 128, 192];
+const SWEEP_RE = [707, 1000, 1414, 2000, 2828, 4000];
 
-/** Requests: Poisson arrivals at `rate`/s (Mulberry32(seed)), a ladder from LADDERS, IC tubes or random (seeded),
- *  Re log-uniform in [707, 4000], and a step budget of 200..1000 steps on the finest rung. */
+/** Requests: Poisson arrivals at `rate`/s (Mulberry32(seed)), IC tubes or random (seeded), a step budget of
+ *  200..1000 steps, and either a refinement ladder from LADDERS at one Re log-uniform in [707, 4000] (the budget is
+ *  the finest rung's), or, with probability `sweepFraction` (default 0), a Reynolds sweep: one grid from SWEEP_GRIDS
+ *  at three consecutive Re of SWEEP_RE. Each request lists its members {N, Re}. */
 function requestSpec(opts = {}) {
   const n = opts.requests || 60, rate = opts.rate || 2, rnd = mulberry32(opts.seed || 1), out = [];
+  const sweepFraction = opts.sweepFraction || 0;
   let t = 0;
   for (let i = 0; i < n; i++) {
     t += -Math.log(1 - rnd()) / rate;
-    const ladder = (opts.ladders || LADDERS)[Math.floor(rnd() * (opts.ladders || LADDERS).length)];
+    const sweep = sweepFraction > 0 && rnd() < sweepFraction;
+    let q;
+    if (sweep) {
+      const grids = opts.sweepGrids || SWEEP_GRIDS;
+      const N = grids[Math.floor(rnd() * grids.length)], k = Math.floor(rnd() * (SWEEP_RE.length - 2));
+      q = { kind: 'sweep', N, Res: SWEEP_RE.slice(k, k + 3) };
+      q.members = q.Res.map(Re => ({ N, Re }));
+    } else {
+      const ladder = (opts.ladders || LADDERS)[Math.floor(rnd() * (opts.ladders || LADDERS).length)];
+      q = { kind: 'ladder', ladder };
+    }
     const ic = rnd() < 0.5 ? 'tubes' : 'random';
     const Re = logUniform(rnd(), 707, 4000), steps = 200 + Math.floor(rnd() * 801);
-    out.push({ id: i, arrival: t, ladder, ic, Re, steps, icParams: ic === 'random' ? { seed: i + 1, k0: 4, E0: 0.5 } : {} });
+    if (!sweep) { q.Re = Re; q.members = q.ladder.map(N => ({ N, Re })); }
+    out.push(Object.assign(q, { id: i, arrival: t, ic, steps, icParams: ic === 'random' ? { seed: i + 1, k0: 4, E0: 0.5 } : {} }));
   }
   return out;
+}
+
+/** A Reynolds sweep's summary from its levels (one per Re, booked as a ladder's): per Re the peaks and E(t_end), the
+ *  least-squares slopes of log peak max|omega| and log peak eps against log Re, and the worst health grade. */
+function sweepSummary(levels) {
+  const x = levels.map(l => Math.log(l.Re));
+  const slope = (y) => {
+    const n = x.length, mx = x.reduce((a, b) => a + b, 0) / n, my = y.reduce((a, b) => a + b, 0) / n;
+    let sxy = 0, sxx = 0;
+    for (let i = 0; i < n; i++) { sxy += (x[i] - mx) * (y[i] - my); sxx += (x[i] - mx) * (x[i] - mx); }
+    return sxx > 0 ? sxy / sxx : null;
+  };
+  const rank = { PASS: 0, WARN: 1, FAIL: 2 };
+  const worst = levels.map(l => l.health || 'PASS').reduce((a, b) => (rank[b] > rank[a] ? b : a), 'PASS');
+  const sOm = slope(levels.map(l => Math.log(l.peak.omMax))), sEps = slope(levels.map(l => Math.log(l.epsPeak.v)));
+  return { Re: levels.map(l => l.Re), omMaxPeak: levels.map(l => l.peak.omMax), epsPeak: levels.map(l => l.epsPeak.v),
+    Eend: levels.map(l => l.Eend), slope_omMax: sOm, slope_eps: sEps, verdict: worst,
+    text: `${levels[0].label} Re ${levels.map(l => l.Re).join(', ')}: d log ω_max / d log Re ${sOm.toFixed(3)}, ` +
+      `d log ε_peak / d log Re ${sEps.toFixed(3)}` };
 }
 
 // Slots per pool of grid N: about `gbPerPool` GB of slots, at most 32.
@@ -150,8 +189,8 @@ function finishRung(r) {
   const s = r.solver.st.series;
   const peak = (arr) => { let i = 0; for (let k = 0; k < arr.length; k++) if (arr[k] > arr[i]) i = k; return { v: arr[i], t: s.t[i] }; };
   const po = peak(s.omMax);
-  return { label: `${r.N}³`, N: r.N, peak: { omMax: po.v, t: po.t }, epsPeak: peak(s.eps), Eend: s.E[s.E.length - 1],
-    steps: r.solver.st.step, t: r.solver.st.t, health: r.solver.health().worst };
+  return { label: `${r.N}³`, N: r.N, Re: r.Re, peak: { omMax: po.v, t: po.t }, epsPeak: peak(s.eps),
+    Eend: s.E[s.E.length - 1], steps: r.solver.st.step, t: r.solver.st.t, health: r.solver.health().worst };
 }
 
 function serve(opts = {}) {
@@ -176,15 +215,16 @@ function serve(opts = {}) {
   }
   function tryAdmit(q) {
     const claimed = new Map(), at = [];
-    for (const N of q.ladder) {
-      const p = claim(N, claimed);
+    for (const m of q.members) {
+      const p = claim(m.N, claimed);
       if (!p) return false;
       claimed.set(p, (claimed.get(p) || 0) + 1);
       at.push(p);
     }
-    // finest rung first: its initial max|u| fixes the common end time (the step budget at CFL 0.5 on that grid)
-    const rungs = q.ladder.map((N, i) => ({ N, p: at[i] }));
-    const mk = r => r.p.pool.admit({ N: r.N, Re: q.Re, cfl, ic: q.ic, icParams: q.icParams });
+    // the last member first (a ladder's finest rung; a sweep's members share grid and IC): its initial max|u| fixes
+    // the common end time (the step budget at CFL 0.5 on that grid)
+    const rungs = q.members.map((m, i) => ({ N: m.N, Re: m.Re, p: at[i] }));
+    const mk = r => r.p.pool.admit({ N: r.N, Re: r.Re, cfl, ic: q.ic, icParams: q.icParams });
     const fine = rungs[rungs.length - 1];
     fine.solver = mk(fine);
     const dt0 = cfl * (2 * Math.PI / fine.N) / Math.max(fine.solver.last.uMax, 1e-9);
@@ -222,17 +262,20 @@ function serve(opts = {}) {
       }
       if (L.left === 0) {
         const levels = L.rungs.map(r => r.done);
-        results[L.q.id] = { id: L.q.id, ladder: L.q.ladder, ic: L.q.ic, Re: L.q.Re, arrival: L.q.arrival,
-          admitted: L.admitted, finished: tDone, latency: tDone - L.q.arrival, tEnd: L.tEnd, levels,
-          summary: NS.studySummary(levels, 'grid') };
+        results[L.q.id] = { id: L.q.id, kind: L.q.kind, ladder: L.q.ladder, N: L.q.N, Res: L.q.Res, ic: L.q.ic,
+          Re: L.q.Re, arrival: L.q.arrival, admitted: L.admitted, finished: tDone, latency: tDone - L.q.arrival,
+          tEnd: L.tEnd, levels,
+          summary: L.q.kind === 'sweep' ? sweepSummary(levels) : NS.studySummary(levels, 'grid') };
         completed++;
         live.splice(i, 1);
       } else i++;
     }
   }
   const wall = now(), lat = results.map(r => r.latency), rungSteps = results.reduce((a, r) => a + r.levels.reduce((b, l) => b + l.steps, 0), 0);
-  return { requests: reqs.length, rate: opts.rate || 2, wall_s: wall, ladders_per_hour: reqs.length / wall * 3600,
-    rung_solves_per_hour: 3 * reqs.length / wall * 3600, p50_latency_s: quantile(lat, 0.5), p99_latency_s: quantile(lat, 0.99),
+  const solves = results.reduce((a, r) => a + r.levels.length, 0), sweeps = results.filter(r => r.kind === 'sweep').length;
+  return { requests: reqs.length, sweeps, rate: opts.rate || 2, wall_s: wall, requests_per_hour: reqs.length / wall * 3600,
+    ladders_per_hour: (reqs.length - sweeps) / wall * 3600, sweeps_per_hour: sweeps / wall * 3600,
+    rung_solves_per_hour: solves / wall * 3600, p50_latency_s: quantile(lat, 0.5), p99_latency_s: quantile(lat, 0.99),
     rung_steps: rungSteps, pools: pools.map(p => ({ N: p.N, slots: p.pool.slots })), reserved_bytes: reserved,
     budget_bytes: budget, results };
 }
@@ -243,10 +286,10 @@ function serveSequential(reqs, results, sample) {
   const t0 = performance.now(), out = [];
   for (const q of reqs.slice(0, sample)) {
     const R = results[q.id], levels = [];
-    for (const N of q.ladder) {
-      const s = NS.createSolver({ N, Re: q.Re, cfl: 0.5, ic: q.ic, icParams: q.icParams });
+    for (const m of q.members) {
+      const s = NS.createSolver({ N: m.N, Re: m.Re, cfl: 0.5, ic: q.ic, icParams: q.icParams });
       s.run(20 * q.steps, R.tEnd);
-      levels.push({ N, steps: s.st.step, t: s.st.t, Eend: s.st.series.E[s.st.series.E.length - 1] });
+      levels.push({ N: m.N, Re: m.Re, steps: s.st.step, t: s.st.t, Eend: s.st.series.E[s.st.series.E.length - 1] });
     }
     out.push({ id: q.id, levels });
   }
@@ -277,7 +320,8 @@ function sequentialEstimate(results, costs) {
   return t;
 }
 
-module.exports = { mulberry32, singleSolveCosts, sequentialEstimate, ensembleSpec, runEnsemble, requestSpec, serve, serveSequential, LADDERS, ENSEMBLE_MIX };
+module.exports = { mulberry32, singleSolveCosts, sequentialEstimate, ensembleSpec, runEnsemble, requestSpec, serve,
+  serveSequential, sweepSummary, LADDERS, SWEEP_GRIDS, SWEEP_RE, ENSEMBLE_MIX };
 
 if (require.main === module) {
   const argv = process.argv.slice(2), mode = argv[0];
@@ -287,11 +331,12 @@ if (require.main === module) {
     out = runEnsemble({ steps: arg('steps', 300), seqPerN: arg('seq-per-n', 8), chunk: arg('chunk', 64) });
   } else if (mode === 'serve') {
     const o = { requests: arg('requests', 60), rate: arg('rate', 2), seed: arg('seed', 1), chunk: arg('chunk', 32),
-      budgetGB: arg('budget-gb', 250) };
+      budgetGB: arg('budget-gb', 250), sweepFraction: arg('sweeps', 0) };
     out = serve(o);
     // sequential single-solve execution of the same work: per-grid costs measured here, times each rung's steps;
     // a sample of requests is also run sequentially for real, to check the estimate and bit-identity
-    const Ns = [...new Set(LADDERS.flat())].sort((a, b) => a - b), costs = singleSolveCosts(Ns);
+    const Ns = [...new Set(LADDERS.flat().concat(o.sweepFraction > 0 ? SWEEP_GRIDS : []))].sort((a, b) => a - b);
+    const costs = singleSolveCosts(Ns);
     const seqAll = sequentialEstimate(out.results, costs);
     Object.assign(out, { single_solve_costs: costs, sequential_estimate_s: seqAll,
       throughput_vs_sequential: seqAll / out.wall_s });
@@ -307,7 +352,10 @@ if (require.main === module) {
       Object.assign(out, { sequential_sample: seq, sequential_sample_s: sq.seconds, sequential_sample_estimate_s: est,
         served_equals_sequential: same });
     }
-    if (!argv.includes('--full')) out.results = out.results.map(r => ({ id: r.id, ladder: r.ladder, latency: r.latency, verdict: r.summary && r.summary.verdict }));
+    if (!argv.includes('--full')) {
+      out.results = out.results.map(r => ({ id: r.id, kind: r.kind, ladder: r.ladder, N: r.N, Res: r.Res,
+        latency: r.latency, verdict: r.summary && r.summary.verdict }));
+    }
   } else {
     process.stderr.write('usage: node serve.js ensemble|serve [options]\n');
     process.exit(2);

@@ -69,7 +69,31 @@ const sameSeries = (a, b) => ['t', 'E', 'Z', 'eps', 'omMax', 'uMax', 'dt', 'ebal
   check('serve: ladder verdicts present', mix.results.every(r => r.summary && ['PASS', 'WARN', 'FAIL'].includes(r.summary.verdict)));
 }
 
-// 4. poolDestroy frees a pool's device memory at once; the handle is then refused
+// 4. Reynolds sweeps beside ladders: every member equals its sequential single solve, and a sweep request served
+//    alone gives the same levels as in the mix
+{
+  const base = { requests: 6, rate: 40, seed: 11, chunk: 16, ladders: [[32, 48, 64]], sweepFraction: 0.5,
+    sweepGrids: [32, 48] };
+  const reqs = SV.requestSpec(base), mix = SV.serve(base);
+  const sweeps = mix.results.filter(r => r.kind === 'sweep');
+  check('sweeps: the mix holds sweeps and ladders', sweeps.length > 0 && sweeps.length < 6, sweeps.length);
+  check('sweeps: three Re per sweep, one grid', sweeps.every(r => r.levels.length === 3 &&
+    r.levels.every(l => l.N === r.N) && r.levels.map(l => l.Re).join() === r.Res.join()));
+  const seq = SV.serveSequential(reqs, mix.results, 6);
+  check('sweeps: every member equals the sequential single solve', seq.requests.every(q => q.levels.every((l, i) => {
+    const m = mix.results[q.id].levels[i];
+    return l.steps === m.steps && l.t === m.t && l.Eend === m.Eend;
+  })));
+  check('sweeps: summary has the Re scaling', sweeps.every(r => r.summary && Number.isFinite(r.summary.slope_omMax) &&
+    ['PASS', 'WARN', 'FAIL'].includes(r.summary.verdict)));
+  const first = reqs.findIndex(q => q.kind === 'sweep');
+  const alone = SV.serve(Object.assign({}, base, { requests: first + 1 }));
+  const a = alone.results[first].levels, b = mix.results[first].levels;
+  check('sweeps: a sweep alone equals the same sweep in the mix',
+    a.every((l, i) => l.steps === b[i].steps && l.Eend === b[i].Eend && l.peak.omMax === b[i].peak.omMax));
+}
+
+// 5. poolDestroy frees a pool's device memory at once; the handle is then refused
 {
   const addon = require(path.join(pkg, '..', 'lib', 'nslab_napi.node'));
   const slotBytes = addon.poolSlotBytes(128);
