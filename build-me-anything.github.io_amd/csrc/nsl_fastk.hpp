@@ -264,6 +264,7 @@ template <> struct ZuSched<1024> { using L = RList<16, 8, 8>; };
 #endif
 // Lines per workgroup: 8 (whole 128-byte rows); 16 at 32^3 (one wave: 4 threads per line, and the row pitch is 16);
 // 4 at 1024^3, where 8 lines would need 1024 threads with at most 128 registers each.
+// (24 lines at 48^3 and 64^3, the whole row pitch, measured 2 % slower in pools)
 __host__ __device__ constexpr int zu_lines(int N) { return N > 512 ? 4 : N == 32 ? 16 : NSL_ZU_T; }
 template <int N> struct ZuCfg {
   static constexpr int T = zu_lines(N), R = 8, NB = N / R, NT = NB * T;
