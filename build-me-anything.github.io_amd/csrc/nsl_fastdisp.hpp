@@ -15,9 +15,9 @@
 #endif
 // Up to 64^3 one tile covers the whole row pitch (16, 24, 24 lines), so a workgroup's 256 threads have work: in a
 // pool of 32 solves the 48^3 step went from 1.29 to 0.98 ms, 32^3 from 0.56 to 0.53; at 96^3 (40 lines) it was 5 %
-// slower.
+// slower. 128^3 takes 16 lines (+3.5 % alone, +3.9 % in a pool); 24 lines at 192^3 were 4.6 % slower alone.
 static constexpr int fast_T(int N) {
-  return N <= 64 ? ((N / 3 + 1) + 7) & ~7 : N <= 256 ? NSL_T_SMALL : 8;
+  return N <= 64 ? ((N / 3 + 1) + 7) & ~7 : N == 128 ? 16 : N <= 256 ? NSL_T_SMALL : 8;
 }
 // Compute units of the current device (cached per device).
 static int device_cus() {
