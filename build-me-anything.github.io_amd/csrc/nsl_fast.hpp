@@ -82,6 +82,19 @@ template <int SIGN> struct Dft<4, SIGN> {
   }
 };
 
+template <int SIGN> struct Dft<6, SIGN> {
+  static __device__ __forceinline__ void run(double2 *v) {
+    // n = 2 n1 + n2, j = j1 + 3 j2: DFT3 over n1, twiddle w6^(j1 n2), DFT2 over n2
+    double2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+    Dft<3, SIGN>::run(e);
+    Dft<3, SIGN>::run(o);
+    o[1] = mul_tw<SIGN>(o[1], make_double2(0.5, 0.8660254037844386));    // w6
+    o[2] = mul_tw<SIGN>(o[2], make_double2(-0.5, 0.8660254037844386));   // w6^2
+#pragma unroll
+    for (int j1 = 0; j1 < 3; j1++) { v[j1] = cadd(e[j1], o[j1]); v[j1 + 3] = csub(e[j1], o[j1]); }
+  }
+};
+
 template <int SIGN> struct Dft<8, SIGN> {
   static __device__ __forceinline__ void run(double2 *v) {
     NSL_FFT_CONTRACT

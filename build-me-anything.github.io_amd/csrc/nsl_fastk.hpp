@@ -614,12 +614,14 @@ template <int N> struct XnlWave;
 template <> struct XnlWave<32> { using L = RList<4, 2, 4>; };
 template <> struct XnlWave<48> { using L = RList<4, 3, 4>; };
 template <> struct XnlWave<64> { using L = RList<8, 8>; };
-template <> struct XnlWave<96> { using L = RList<4, 3, 2, 4>; };
+// 96^3: 4.6.4 (radix-6 middle stage; 4.3.2.4 took one LDS exchange more: 29.3 against 26.2 us alone)
+template <> struct XnlWave<96> { using L = RList<4, 6, 4>; };
 template <> struct XnlWave<128> { using L = RList<8, 2, 8>; };
 // 192^3 and 384^3 with radix-4 ends (4.3.4.4, 4.4.3.4; 4.3.8.4, 4.8.3.4) need fewer registers (166-208) but add an
-// LDS exchange: 150-205 us against 140-146 us at 192^3, 1646-1652 against 1477 us at 384^3
+// LDS exchange: 150-205 us against 140-146 us at 192^3, 1646-1652 against 1477 us (8.3.2.8) at 384^3
 template <> struct XnlWave<192> { using L = RList<8, 3, 8>; };
-template <> struct XnlWave<384> { using L = RList<8, 3, 2, 8>; };
+// 384^3: 8.6.8 (8.3.2.8: 1468 against 1192 us)
+template <> struct XnlWave<384> { using L = RList<8, 6, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
 template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
 template <int N>
