@@ -153,7 +153,7 @@ template <> struct Sched<24> { using L = RList<8, 3>; };
 template <> struct Sched<32> { using L = RList<8, 4>; };
 template <> struct Sched<48> { using L = RList<16, 3>; };
 template <> struct Sched<64> { using L = RList<8, 8>; };
-template <> struct Sched<96> { using L = RList<8, 4, 3>; };
+template <> struct Sched<96> { using L = RList<8, 4, 3>; };   // 16.6 (one exchange fewer): 19 % slower in pools
 template <> struct Sched<128> { using L = RList<16, 8>; };
 template <> struct Sched<192> { using L = RList<16, 4, 3>; };
 template <> struct Sched<256> { using L = RList<16, 16>; };
