@@ -247,7 +247,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zfwd(const double2 *__restric
 // Sums: one partial per workgroup (fixed-order tree), grid ntile*by.
 template <int N> struct ZuSched;
 template <> struct ZuSched<32> { using L = RList<4, 8>; };
-template <> struct ZuSched<48> { using L = RList<2, 3, 8>; };
+template <> struct ZuSched<48> { using L = RList<2, 3, 8>; };   // 6.8: no difference in pools or alone
 template <> struct ZuSched<64> { using L = RList<8, 8>; };
 template <> struct ZuSched<96> { using L = RList<4, 3, 8>; };
 template <> struct ZuSched<128> { using L = RList<16, 8>; };
