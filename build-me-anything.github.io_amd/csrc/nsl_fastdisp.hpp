@@ -69,6 +69,7 @@ static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, 
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   if (mode == 1 && nout == 6 && N <= NSL_ZINV_UW_MAXN) {   // the rhs: u and omega from five transforms per tile
+    // (4-line tiles at 1024^3, two workgroups per CU: 12.1 against 8.8 ms)
     allow_lds(fast::kf_zinv_uw<N, T>, lds);
     hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by), 1, bt.nz), dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl, bt.sb);
     LAUNCH_CHECK();
