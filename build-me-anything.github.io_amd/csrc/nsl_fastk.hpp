@@ -736,6 +736,7 @@ template <int N> struct XnlStaged {
   static constexpr int R = FirstR<typename XnlWave<N>::L>::R, NB = N / R;
   static constexpr int NT = NB / gcd_int(NB, 64) * 64, RPW = NT / NB, WPR = NT / 64;
   // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
+  // (three waves per SIMD, 168 registers, spills 95 at 192^3 and 111 at 384^3)
   static constexpr int MINW = NT == 128 ? NSL_XNL_BIG_MINW : NT > 64 ? 2 : 1;
   // one-wave groups hold the twiddles of all five transforms in registers (TwCache); multi-wave groups read them
   // from the table at each stage: at 1024^3 (8.8.2.8) the cache is the difference between 248 and 300+ registers
