@@ -161,8 +161,9 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Max of non-negative doubles over the workgroup into *dst (atomicMax on the IEEE bit pattern: for x >= 0 the
 // unsigned order of the bits is the numeric order). Exact, so independent of scheduling.
+template <int NT = NSL_NT>
 __device__ __forceinline__ void block_atomic_max2(double a, double b, unsigned long long *da,
-                                                  unsigned long long *db, double *scratch) {
+                                                  unsigned long long *db, double *scratch /* 2*NT/64 */) {
   a = wave_max(a); b = wave_max(b);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
@@ -170,7 +171,7 @@ __device__ __forceinline__ void block_atomic_max2(double a, double b, unsigned l
   __syncthreads();
   if (threadIdx.x == 0) {
     double ma = scratch[0], mb = scratch[1];
-    for (int w = 1; w < NSL_NT / 64; w++) { ma = fmax(ma, scratch[2 * w]); mb = fmax(mb, scratch[2 * w + 1]); }
+    for (int w = 1; w < NT / 64; w++) { ma = fmax(ma, scratch[2 * w]); mb = fmax(mb, scratch[2 * w + 1]); }
     atomicMax(da, (unsigned long long)__double_as_longlong(ma));
     atomicMax(db, (unsigned long long)__double_as_longlong(mb));
   }

@@ -91,11 +91,15 @@ static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp
   return NSL_OK;
 }
 template <int N>
-static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt) {
+static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
+                       const double *pmax, int nblk) {
   constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_yfwd<N, T>, lds);
-  hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X3, X4, x.g, x.twN, ctl, bt.sb);
+  const unsigned grid = (unsigned)(fast_ntile(x) * N);
+  // workgroups that combine the x-pass maxima: about 8 KB of them each, at most the grid's first row
+  const int nred = pmax && nblk > 0 ? (int)std::min<long long>(grid, nblk / fast::PassNT<N>::v + 1) : 0;
+  hipLaunchKernelGGL((fast::kf_yfwd<N, T>), dim3(grid, (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X3, X4, x.g, x.twN, ctl, bt.sb, pmax, nblk, nred);
   LAUNCH_CHECK();
   return NSL_OK;
 }
@@ -141,7 +145,9 @@ static int fast_xnl_pair(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, co
 
 // pmax: 2 * (workgroups of the x pass) doubles per slot, at most 2 N^2
 template <int N>
-static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
+// The x pass leaves one maxima pair per workgroup in pmax; *nred_out is their number, which the y-forward pass
+// combines (fast_yfwd_n).
+static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out) {
   const TuneCfg &c = x.tune;
   int nblk = 0;
   switch (c.xpass) {
@@ -187,8 +193,7 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
   if (nblk == 0) return fail(NSL_ERR_STATE, "x-pass form " + std::to_string(c.xpass) + "/" + std::to_string(c.lp) + " is not compiled for N = " + std::to_string(N));
   if (nblk > N * N) return fail(NSL_ERR_STATE, "x-pass maxima exceed their buffer");   // pmax holds N^2 pairs
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(fast::k_reduce_max, dim3((unsigned)std::min(nblk / NSL_NT + 1, 512), 1, bt.nz), dim3(NSL_NT), 0, x.st, (const double *)pmax, nblk, ctl, bt.sb);
-  LAUNCH_CHECK();
+  *nred_out = nblk;   // the maxima pairs the y-forward pass combines (kf_yfwd)
   return NSL_OK;
 }
 
@@ -216,14 +221,15 @@ static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int
 static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl, bt)
 }
-static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt) {
-  NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl, bt)
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
+                              const double *pmax, int nblk) {
+  NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl, bt, pmax, nblk)
 }
 static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_zfwd_n, X4, OUT, ncomp, ctl, bt)
 }
-static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
-  NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt)
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out) {
+  NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt, nred_out)
 }
 
 // z-forward fused with the stage-`stage` update (kf_zfwd_upd) where it is compiled (zu_has) and chosen

@@ -195,9 +195,25 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
 // P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
 template <int N, int T>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
-                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb,
+                                                   const double *__restrict__ pmax, int nblk, int nred) {
   X3 = at_slot(X3, sb); X4 = at_slot(X4, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
+  // The x pass's per-workgroup maxima (max |omega|^2, max |u|^2), when given: the first nred workgroups of component 0
+  // combine them into the control block, in place of a reduction kernel of their own. The x pass has completed before
+  // this pass starts, and the update that reads the maxima starts after it.
+  if (pmax != nullptr && ctl != nullptr && blockIdx.y == 0 && (int)blockIdx.x < nred) {
+    constexpr int NT = PassNT<N>::v;
+    __shared__ double mscr[2 * (NT / 64)];
+    pmax = at_slot(pmax, sb);
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < nblk; i += nred * NT) {
+      a = fmax(a, pmax[2 * (size_t)i]);
+      b = fmax(b, pmax[2 * (size_t)i + 1]);
+    }
+    Ctl *cw = const_cast<Ctl *>(ctl);
+    block_atomic_max2<NT>(a, b, &cw->om2, &cw->u2, mscr);
+  }
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
@@ -886,21 +902,6 @@ __global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_d
   } else {
     if (lane == 0) { pmax[2 * (size_t)blockIdx.x] = om2; pmax[2 * (size_t)blockIdx.x + 1] = u2; }
   }
-}
-
-// Reduces the per-workgroup maxima of the x pass (pmax[2*b], pmax[2*b+1]) into ctl->om2, ctl->u2: a few hundred
-// workgroups, one atomic pair each (the max is exact, so the order does not matter).
-__global__ __launch_bounds__(NSL_NT) void k_reduce_max(const double *__restrict__ pmax, int nblk, Ctl *__restrict__ ctl,
-                                                        size_t sb) {
-  __shared__ double scratch[8];
-  pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
-  if (ctl->active == 0) return;
-  double a = 0.0, b = 0.0;
-  for (int i = blockIdx.x * NSL_NT + threadIdx.x; i < nblk; i += gridDim.x * NSL_NT) {
-    a = fmax(a, pmax[2 * (size_t)i]);
-    b = fmax(b, pmax[2 * (size_t)i + 1]);
-  }
-  block_atomic_max2(a, b, &ctl->om2, &ctl->u2, scratch);
 }
 
 }  // namespace fast

@@ -161,9 +161,10 @@ struct Batch {
 
 static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt);
 static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt);
-static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
+                              const double *pmax, int nblk);
 static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt);
-static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt);
+static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out);
 static bool fast_setup(Xform &x);
 
 static Geo geo_truncated(int N) {
@@ -228,8 +229,10 @@ static int yfwd_t(const Xform &x, const double2 *X3, double2 *X4, int ncomp, con
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt = Batch()) {
-  if (x.fast) return fast_dispatch_yfwd(x, X3, X4, ncomp, ctl, bt);
+// pmax, nblk: x-pass maxima still to combine into ctl (the fast path's x pass leaves them; see xnl), or none
+static int yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt = Batch(),
+                const double *pmax = nullptr, int nblk = 0) {
+  if (x.fast) return fast_dispatch_yfwd(x, X3, X4, ncomp, ctl, bt, pmax, nblk);
   TRY(need_single(bt));
   return x.tl.TNp == 4096 ? yfwd_t<4096>(x, X3, X4, ncomp, ctl) : yfwd_t<8192>(x, X3, X4, ncomp, ctl);
 }
@@ -262,8 +265,10 @@ static int xnl_t(const Xform &x, double2 *X2, Ctl *ctl) {
   LAUNCH_CHECK();
   return NSL_OK;
 }
-static int xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt = Batch()) {
-  if (x.fast) return fast_dispatch_xnl(x, X2, pmax, ctl, bt);
+// *nred: the number of maxima pairs left in pmax for the next y-forward pass to combine (0: already in ctl)
+static int xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, int *nred, const Batch &bt = Batch()) {
+  *nred = 0;
+  if (x.fast) return fast_dispatch_xnl(x, X2, pmax, ctl, bt, nred);
   TRY(need_single(bt));
   return x.tl.TNx == 4096 ? xnl_t<4096>(x, X2, ctl) : xnl_t<8192>(x, X2, ctl);
 }
@@ -454,8 +459,9 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
   const Batch &bt = s->bt;
   TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, bt)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt)); TRY(prof_end(s));
+  int nred = 0;
+  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, &nred, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt, s->pmax, nred)); TRY(prof_end(s));
   int nsum = s->nblk;   // partials the finalize sums
   if (fast_has_zu(x)) {
     // z-forward and update in one pass: the nonlinear term stays on chip (timed as the update of this stage)
