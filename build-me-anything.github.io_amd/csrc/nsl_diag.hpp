@@ -208,6 +208,30 @@ __global__ void k_exact(const double *__restrict__ ph, int N, int kind, double A
   }
 }
 
+// N x N plane of a physical quantity as float32 (slice, nslab.js:566-583). kind: NSL_SLICE_* (|omega|, |u|, kinetic
+// energy density, Q, stretching, u, v, w, omega_x..z; other kinds give zeros). The plane is x = ix (axis 0), y = ix
+// (axis 1) or z = ix (axis 2); out[b*N + a] walks the two remaining coordinates, the lower one fastest.
+__global__ void k_slice(const double *__restrict__ ph, int N, int kind, int axis, int ix, float *__restrict__ out) {
+  const size_t N3 = (size_t)N * N * N;
+  const int nn = N * N;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nn; q += gridDim.x * blockDim.x) {
+    const int a = q % N, b = q / N;
+    const size_t n = axis == 2 ? ((size_t)ix * N + b) * N + a : axis == 1 ? ((size_t)b * N + ix) * N + a
+                                                                         : ((size_t)b * N + a) * N + ix;
+    double v = 0.0;
+    if (kind <= 2) {   // magnitudes: of omega (kind 0) or u (1, 2)
+      const double *f = ph + (kind == 0 ? PH_OM : PH_U) * N3;
+      const double x = f[n], y = f[N3 + n], z = f[2 * N3 + n];
+      const double sq = x * x + y * y + z * z;
+      v = kind == 2 ? 0.5 * sq : sqrt(sq);
+    } else if (kind == 3) v = ph[PH_Q * N3 + n];
+    else if (kind == 4) v = ph[PH_PST * N3 + n];
+    else if (kind <= 7) v = ph[(size_t)(PH_U + kind - 5) * N3 + n];
+    else if (kind <= 10) v = ph[(size_t)(PH_OM + kind - 8) * N3 + n];
+    out[q] = (float)v;
+  }
+}
+
 // |omega|^2 on the grid (interpMax start nodes).
 __global__ void k_o2(const double *__restrict__ ph, size_t N3, double *__restrict__ o2) {
   const double *ox = ph + PH_OM * N3, *oy = ph + (PH_OM + 1) * N3, *oz = ph + (PH_OM + 2) * N3;
@@ -334,42 +358,58 @@ static int spectrum_dev(nsl_solver *s, double *out) {
 
 static int spectrum_impl(nsl_solver *s, double *Ek) { return spectrum_dev(s, Ek); }
 
-// cutoffPileUp (nslab.js:557-564): 1 when defined.
+// cutoffPileUp (nslab.js:557-564): how far the spectrum rises between k8 = round(0.8 kc) and the cutoff kc, as the
+// largest E(k)/E(k8) over that band (>= 1 by construction). Undefined (returns 0) when E(k8) is not above the FP64
+// noise floor of a quadratic quantity, 100 eps^2 times the largest E(k), k >= 1; returns 1 with the value otherwise.
 static int pile_up(int kc, const double *spec, int nb, double *out) {
+  const double eps = 2.220446049250313e-16;   // FP64 machine epsilon
   const int k8 = (int)floor(0.8 * kc + 0.5);
-  double peak = 0;
-  for (int k = 1; k < nb; k++) peak = std::max(peak, spec[k]);
-  const double SPEC_FLOOR = 100 * 2.220446049250313e-16 * 2.220446049250313e-16;
-  if (!(spec[k8] > SPEC_FLOOR * peak)) return 0;
-  double m = 0;
-  for (int k = k8; k <= kc; k++) m = std::max(m, spec[k] / spec[k8]);
-  *out = m;
+  const double base = spec[k8];
+  const double peak = nb > 1 ? *std::max_element(spec + 1, spec + nb) : 0.0;
+  if (!(base > 100 * eps * eps * std::max(peak, 0.0))) return 0;
+  double rise = 1.0;                          // E(k8)/E(k8)
+  for (int k = k8 + 1; k <= kc; k++) rise = std::max(rise, spec[k] / base);
+  *out = rise;
   return 1;
 }
 
-// imageDiag's host bookkeeping over the N-value profile (nslab.js:537-550).
+// imageDiag's bookkeeping over the enstrophy profile Ez[0..N) along z (nslab.js:537-550). The profile is a ring:
+// o[0] the circular centroid (angle of sum Ez[l] e^{i z_l}, in [0, 2 pi)), o[3] the number of maximal circular runs
+// of planes above 1 % of the profile's maximum, o[1] the longest run's length in z (2 pi when no plane is above:
+// nothing localised), o[2] the gap 2 pi - o[1] that run leaves to its own periodic image.
 static void image_diag(const std::vector<double> &Ez, int N, double *o) {
   double mx = 0, cs = 0, sn = 0;
-  for (int l = 0; l < N; l++) { mx = std::max(mx, Ez[l]); const double z = l * TWO_PI / N; cs += Ez[l] * cos(z); sn += Ez[l] * sin(z); }
-  const double zc = fmod(fmod(atan2(sn, cs), TWO_PI) + TWO_PI, TWO_PI);
-  std::vector<int> on(N);
-  for (int l = 0; l < N; l++) on[l] = Ez[l] > 0.01 * mx;
-  std::vector<std::pair<int, int>> runs, rr;
-  int start = -1;
   for (int l = 0; l < N; l++) {
-    if (on[l] && start < 0) start = l;
-    if (!on[l] && start >= 0) { runs.push_back({start, l}); start = -1; }
+    const double z = l * TWO_PI / N;
+    mx = std::max(mx, Ez[l]);
+    cs += Ez[l] * cos(z);
+    sn += Ez[l] * sin(z);
   }
-  if (start >= 0) runs.push_back({start, N});
-  if (runs.size() > 1 && on[0] && on[N - 1]) {
-    rr.push_back({runs.back().first, runs[0].second + N});
-    for (size_t q = 1; q + 1 < runs.size(); q++) rr.push_back(runs[q]);
-  } else rr = runs;
-  int mxl = 0;
-  for (auto &r : rr) mxl = std::max(mxl, r.second - r.first);
-  double ext = (rr.size() ? mxl : N) * TWO_PI / N;
-  ext = std::min(ext, TWO_PI);
-  o[0] = zc; o[1] = ext; o[2] = std::max(TWO_PI - ext, 0.0); o[3] = (double)rr.size();
+  o[0] = fmod(fmod(atan2(sn, cs), TWO_PI) + TWO_PI, TWO_PI);
+  const double thr = 0.01 * mx;
+  auto above = [&](int l) { return Ez[((l % N) + N) % N] > thr; };
+  int first_off = -1;
+  for (int l = 0; l < N && first_off < 0; l++)
+    if (!above(l)) first_off = l;
+  int bands = 0, longest = 0;
+  if (first_off < 0) {             // every plane above: one band around the whole ring
+    bands = 1;
+    longest = N;
+  } else {                         // walk the ring once, starting just after a plane below the threshold
+    int len = 0;
+    for (int q = 1; q <= N; q++) {
+      if (above(first_off + q)) {
+        if (len == 0) bands++;
+        longest = std::max(longest, ++len);
+      } else {
+        len = 0;
+      }
+    }
+  }
+  const double ext = std::min((bands ? longest : N) * TWO_PI / N, TWO_PI);
+  o[1] = ext;
+  o[2] = std::max(TWO_PI - ext, 0.0);
+  o[3] = (double)bands;
 }
 
 // interpMax (nslab.js:421-536): start nodes chosen from the grid |omega|^2 (stable order, as the reference's sort),
@@ -606,6 +646,37 @@ static int field_impl(nsl_solver *s, int which, double *out) {
   const int slot = which < 6 ? which : (which == 6 ? PH_Q : PH_PST);
   HIP_TRY(hipMemcpyAsync(out, s->phys + (size_t)slot * N3, N3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
+  return NSL_OK;
+}
+
+static int slice_impl(nsl_solver *s, int kind, int axis, int index, float *out) {
+  const int N = s->N;
+  TRY(ensure_phys(s));
+  if (!s->diag_valid && (kind <= 2 || (kind >= 5 && kind <= 10))) TRY(physical_u_om(s));
+  const int ix = std::max(0, std::min(N - 1, index));
+  TmpBuf d;
+  HIP_TRY(hipMalloc(&d.p, (size_t)N * N * sizeof(float)));
+  hipLaunchKernelGGL(k_slice, dim3((unsigned)std::min(4096, (N * N + NSL_NT - 1) / NSL_NT)), dim3(NSL_NT), 0, s->st,
+                     (const double *)s->phys, N, kind, axis, ix, (float *)d.p);
+  LAUNCH_CHECK();
+  HIP_TRY(hipMemcpyAsync(out, d.p, (size_t)N * N * sizeof(float), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  return NSL_OK;
+}
+
+// imageDiag() of the current state: the vorticity's enstrophy profile along z on the GPU, then image_diag.
+static int image_diag_impl(nsl_solver *s, double *o) {
+  const int N = s->N;
+  TRY(ensure_phys(s));
+  if (!s->diag_valid) TRY(physical_u_om(s));
+  TmpBuf ezb;
+  HIP_TRY(hipMalloc(&ezb.p, (size_t)N * sizeof(double)));
+  hipLaunchKernelGGL(k_zprofile, dim3((unsigned)N), dim3(NSL_NT), 0, s->st, s->phys, N, (double *)ezb.p);
+  LAUNCH_CHECK();
+  std::vector<double> ez(N);
+  HIP_TRY(hipMemcpyAsync(ez.data(), ezb.p, N * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipStreamSynchronize(s->st));
+  image_diag(ez, N, o);
   return NSL_OK;
 }
 

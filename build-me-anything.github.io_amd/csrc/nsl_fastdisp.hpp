@@ -130,43 +130,76 @@ static constexpr bool zu_has(int N) {
          N == 1024;
 }
 
-template <int N, int LP>
-static int fast_xnl_pair(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt) {
-  if constexpr (xp_pair_lp_ok(N, LP)) {
-    const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
-    const long long npairs = (long long)N * (N / 2);
-    allow_lds(fast::kf_xnl<N, LP>, lds);
-    const int nblk = (int)((npairs + LP - 1) / LP);
-    hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)nblk, 1, bt.nz), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-    return nblk;
-  }
-  return 0;
-}
-
-// pmax: 2 * (workgroups of the x pass) doubles per slot, at most 2 N^2
+// Workgroups the x pass of form c launches at grid N on a device of `cus` compute units; each leaves one maxima pair
+// (max |omega|^2, max |u|^2) in pmax. 0 when the form is not compiled for N.
 template <int N>
-// The x pass leaves one maxima pair per workgroup in pmax; *nred_out is their number, which the y-forward pass
-// combines (fast_yfwd_n).
-static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out) {
-  const TuneCfg &c = x.tune;
-  int nblk = 0;
+static long long xnl_workgroups(const TuneCfg &c, int cus) {
   switch (c.xpass) {
     case NSL_XP_STAGED:
       if constexpr (xp_has(N, NSL_XP_STAGED)) {
         using C = fast::XnlStaged<N>;
-        int grid = N * N / C::RPW;
-        // persistent: xwaves waves per CU walk the rows
-        if (c.xwaves > 0) grid = std::min(grid, std::max(1, device_cus() * c.xwaves / C::WPR));
+        long long grid = (long long)N * N / C::RPW;
+        // persistent: xwaves waves per CU walk the row groups
+        if (c.xwaves > 0) grid = std::min<long long>(grid, std::max(1, cus * c.xwaves / C::WPR));
+        return grid;
+      }
+      return 0;
+    case NSL_XP_WAVE:
+      if constexpr (xp_has(N, NSL_XP_WAVE)) return (long long)N * N / (64 / (N / 8));
+      return 0;
+    case NSL_XP_ROWS:
+      if constexpr (xp_has(N, NSL_XP_ROWS)) return (long long)N * N;   // one row per workgroup
+      return 0;
+    case NSL_XP_PAIR:
+      if (c.lp == 1 || c.lp == 2 || c.lp == 4 || c.lp == 8) {
+        const bool ok = c.lp == 1 ? xp_pair_lp_ok(N, 1) : c.lp == 2 ? xp_pair_lp_ok(N, 2) : c.lp == 4 ? xp_pair_lp_ok(N, 4) : xp_pair_lp_ok(N, 8);
+        if (ok) return ((long long)N * (N / 2) + c.lp - 1) / c.lp;
+      }
+      return 0;
+    default:
+      return 0;
+  }
+}
+
+// The x pass's launch plan, checked before anything is enqueued: the form must be compiled for N and its
+// workgroups' maxima pairs must fit the cap_pairs the slot's pmax buffer holds (N^2 pairs).
+template <int N>
+static int xnl_plan(const TuneCfg &c, int cus, long long cap_pairs, long long *grid) {
+  const long long g = xnl_workgroups<N>(c, cus);
+  if (g <= 0) return fail(NSL_ERR_STATE, "x-pass form " + std::to_string(c.xpass) + "/" + std::to_string(c.lp) + " is not compiled for N = " + std::to_string(N));
+  if (g > cap_pairs) return fail(NSL_ERR_STATE, "x-pass maxima exceed their buffer (" + std::to_string(g) + " workgroups, " + std::to_string(cap_pairs) + " pairs)");
+  *grid = g;
+  return NSL_OK;
+}
+
+template <int N, int LP>
+static void fast_xnl_pair_launch(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, long long grid) {
+  if constexpr (xp_pair_lp_ok(N, LP)) {
+    const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
+    allow_lds(fast::kf_xnl<N, LP>, lds);
+    hipLaunchKernelGGL((fast::kf_xnl<N, LP>), dim3((unsigned)grid, 1, bt.nz), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+  }
+}
+
+// The x pass leaves one maxima pair per workgroup in pmax (N^2 pairs per slot); *nred_out is their number, which the
+// y-forward pass combines (fast_yfwd_n). The plan is checked before the launch.
+template <int N>
+static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out) {
+  const TuneCfg &c = x.tune;
+  long long grid = 0;
+  TRY(xnl_plan<N>(c, device_cus(), (long long)N * N, &grid));
+  switch (c.xpass) {
+    case NSL_XP_STAGED:
+      if constexpr (xp_has(N, NSL_XP_STAGED)) {
+        using C = fast::XnlStaged<N>;
         hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)grid, 1, bt.nz), dim3(C::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-        nblk = grid;   // one maxima pair per workgroup
       }
       break;
     case NSL_XP_WAVE:
       if constexpr (xp_has(N, NSL_XP_WAVE)) {
         constexpr int RPW = 64 / (N / 8);
-        nblk = N * N / RPW;
         const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
-        hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)nblk, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        hipLaunchKernelGGL((fast::kf_xnl_wave<N>), dim3((unsigned)grid, 1, bt.nz), dim3(64), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
       }
       break;
     case NSL_XP_ROWS:
@@ -174,26 +207,20 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
         constexpr int RP = 1, NT = 192;   // 3 lines of N/8 or N/16 first-stage butterflies: one per thread
         const int lds = N * 3 * RP * (int)sizeof(double2);
         allow_lds(fast::kf_xnl_rows<N, RP, NT>, lds);
-        nblk = (N * N + RP - 1) / RP;
-        hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)nblk, 1, bt.nz), dim3(NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT>), dim3((unsigned)grid, 1, bt.nz), dim3(NT), lds, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
       }
       break;
-    case NSL_XP_PAIR:
+    default:   // NSL_XP_PAIR (the plan accepted lp)
       switch (c.lp) {
-        case 1: nblk = fast_xnl_pair<N, 1>(x, X2, pmax, ctl, bt); break;
-        case 2: nblk = fast_xnl_pair<N, 2>(x, X2, pmax, ctl, bt); break;
-        case 4: nblk = fast_xnl_pair<N, 4>(x, X2, pmax, ctl, bt); break;
-        case 8: nblk = fast_xnl_pair<N, 8>(x, X2, pmax, ctl, bt); break;
-        default: break;
+        case 1: fast_xnl_pair_launch<N, 1>(x, X2, pmax, ctl, bt, grid); break;
+        case 2: fast_xnl_pair_launch<N, 2>(x, X2, pmax, ctl, bt, grid); break;
+        case 4: fast_xnl_pair_launch<N, 4>(x, X2, pmax, ctl, bt, grid); break;
+        default: fast_xnl_pair_launch<N, 8>(x, X2, pmax, ctl, bt, grid); break;
       }
-      break;
-    default:
       break;
   }
-  if (nblk == 0) return fail(NSL_ERR_STATE, "x-pass form " + std::to_string(c.xpass) + "/" + std::to_string(c.lp) + " is not compiled for N = " + std::to_string(N));
-  if (nblk > N * N) return fail(NSL_ERR_STATE, "x-pass maxima exceed their buffer");   // pmax holds N^2 pairs
   LAUNCH_CHECK();
-  *nred_out = nblk;   // the maxima pairs the y-forward pass combines (kf_yfwd)
+  *nred_out = (int)grid;   // the maxima pairs the y-forward pass combines (kf_yfwd)
   return NSL_OK;
 }
 
@@ -230,6 +257,14 @@ static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, i
 }
 static int fast_dispatch_xnl(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const Batch &bt, int *nred_out) {
   NSL_DISPATCH(fast_xnl_n, X2, pmax, ctl, bt, nred_out)
+}
+static int xnl_plan_n(int N, const TuneCfg &c, int cus, long long cap_pairs, long long *grid) {
+  switch (N) {
+#define NSL_CASE(n) case n: return xnl_plan<n>(c, cus, cap_pairs, grid);
+    NSL_FAST_SIZES(NSL_CASE)
+#undef NSL_CASE
+    default: return fail(NSL_ERR_ARG, "no size-specialised kernels for N = " + std::to_string(N));
+  }
 }
 
 // z-forward fused with the stage-`stage` update (kf_zfwd_upd) where it is compiled (zu_has) and chosen

@@ -357,6 +357,8 @@ static int nblk_of(const Geo &g) { return (int)((g.D + (long long)NSL_NT * NSL_M
 
 struct nsl_solver {
   int N = 0, ic = 0;
+  nsl_pool *pool = nullptr;   // a slot view: its pool and slot index
+  int slot = -1;
   double nu = 0, cfl = 0, dtFixed = 0, icp[4] = {0, 0, 0, 0};
   Xform x;
   hipStream_t st = 0;
@@ -801,9 +803,11 @@ static void pool_free(nsl_pool *p) {
   delete p;
 }
 
-static nsl_solver *pool_view(nsl_pool *p, char *base, Batch bt) {
+static nsl_solver *pool_view(nsl_pool *p, char *base, Batch bt, int slot) {
   nsl_solver *v = new nsl_solver();
   v->view = true;
+  v->pool = p;
+  v->slot = slot;
   v->N = p->N;
   v->x = p->x;
   v->st = p->st;
@@ -842,8 +846,8 @@ static int pool_create_impl(int N, int C, nsl_pool **out) {
     pool_free(p);
     return fail(NSL_ERR_HIP, "staging allocation failed");
   }
-  p->lead = pool_view(p, p->base, Batch{(unsigned)C, p->L.bytes});
-  for (int i = 0; i < C; i++) p->slots.push_back(pool_view(p, p->base + (size_t)i * p->L.bytes, Batch()));
+  p->lead = pool_view(p, p->base, Batch{(unsigned)C, p->L.bytes}, -1);
+  for (int i = 0; i < C; i++) p->slots.push_back(pool_view(p, p->base + (size_t)i * p->L.bytes, Batch(), i));
   p->used.assign(C, 0);
   p->gen.assign(C, 0);
   p->launch_gen.assign(C, 0);
@@ -865,6 +869,17 @@ static int pool_create_impl(int N, int C, nsl_pool **out) {
   }
   (void)hipGetLastError();
   *out = p;
+  return NSL_OK;
+}
+
+// A slot that took part in the chunk still in flight (launched, not yet collected): its control block and state are
+// the chunk's until pool_collect books them, so nothing may re-target, step or overwrite it before then. A slot
+// admitted or re-targeted after the launch was not part of it (its generation moved on).
+static int slot_in_flight(const nsl_solver *s) {
+  if (!s->view || !s->pool || s->slot < 0) return NSL_OK;
+  const nsl_pool *p = s->pool;
+  if (p->lastChunk > 0 && p->gen[s->slot] == p->launch_gen[s->slot])
+    return fail(NSL_ERR_STATE, "slot " + std::to_string(s->slot) + " is in the pool's chunk in flight: collect the pool first");
   return NSL_OK;
 }
 
@@ -917,6 +932,7 @@ static int pool_release_impl(nsl_pool *p, int slot) {
 static int pool_set_target_impl(nsl_pool *p, int slot, int64_t n, int has_tend, double tend) {
   TRY(pool_slot_check(p, slot, true));
   nsl_solver *v = p->slots[slot];
+  TRY(slot_in_flight(v));
   TRY(pull_ctl(v));
   Ctl *c = v->hctl;
   c->nDone = 0; c->nTarget = n > 0 ? n : 0; c->hasTEnd = has_tend ? 1 : 0; c->tEnd = tend; c->nChunk = 0;
@@ -1111,6 +1127,7 @@ int nsl_create(const nsl_opts *opts, nsl_solver **out) { return create_impl(opts
 
 int nsl_set_ic(nsl_solver *s, int32_t ic, const double icp[4]) {
   if (!s) return fail(NSL_ERR_ARG, "null solver");
+  TRY(slot_in_flight(s));
   return set_ic(s, ic, icp);
 }
 
@@ -1121,6 +1138,7 @@ void nsl_destroy(nsl_solver *s) {
 
 int nsl_run(nsl_solver *s, int64_t n, int32_t has_tend, double tend, nsl_steprec *recs, int64_t *n_done) {
   if (!s) return fail(NSL_ERR_ARG, "null solver");
+  TRY(slot_in_flight(s));
   return run_impl(s, n, has_tend, tend, recs, n_done);
 }
 
@@ -1162,6 +1180,7 @@ int nsl_get_state(nsl_solver *s, double *const planes[6]) {
 int nsl_set_state(nsl_solver *s, const double *const planes[6]) {
   if (!s || !planes) return fail(NSL_ERR_ARG, "null argument");
   for (int i = 0; i < 6; i++) if (!planes[i]) return fail(NSL_ERR_ARG, "null plane");
+  TRY(slot_in_flight(s));
   return set_state_impl(s, planes);
 }
 
@@ -1240,6 +1259,27 @@ int nsl_exact_error(nsl_solver *s, double out[2]) {
   return exact_error_impl(s, out);
 }
 
+int nsl_slice(nsl_solver *s, int32_t kind, int32_t axis, int32_t index, float *out) {
+  if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  return slice_impl(s, kind, axis, index, out);
+}
+
+int nsl_image_diag(nsl_solver *s, double out[4]) {
+  if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  return image_diag_impl(s, out);
+}
+
+int nsl_cutoff_pile_up(int32_t kc, const double *spec, int32_t nb, double *out) {
+  if (!spec || !out) return fail(NSL_ERR_ARG, "null argument");
+  if (kc < 1 || nb <= kc) return fail(NSL_ERR_ARG, "spectrum of " + std::to_string(nb) + " bins does not reach kc = " + std::to_string(kc));
+  return pile_up(kc, spec, nb, out);
+}
+
+int nsl_grid_size(const nsl_solver *s) {
+  if (!s) return fail(NSL_ERR_ARG, "null solver");
+  return s->N;
+}
+
 int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
   size_t f = 0, t = 0;
   HIP_TRY(hipMemGetInfo(&f, &t));
@@ -1287,6 +1327,16 @@ int nsl_tune_candidates(int32_t N, nsl_tune *out, int32_t cap) {
   const std::vector<TuneCfg> v = tune_candidates(N);
   if (out) for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = to_abi(v[i]);
   return (int)v.size();
+}
+
+int nsl_xpass_plan(int32_t N, const nsl_tune *cfg, int32_t cus, int64_t cap_pairs, int64_t *workgroups) {
+  if (!cfg || !workgroups) return fail(NSL_ERR_ARG, "null argument");
+  TuneCfg c;
+  c.xpass = cfg->xpass; c.lp = cfg->lp; c.xwaves = cfg->xwaves; c.zfused = cfg->zfused;
+  long long g = 0;
+  TRY(xnl_plan_n(N, c, cus > 0 ? cus : device_cus(), cap_pairs, &g));
+  *workgroups = g;
+  return NSL_OK;
 }
 
 void nsl_pool_destroy(nsl_pool *p) { pool_free(p); }

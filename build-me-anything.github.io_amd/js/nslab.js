@@ -3,17 +3,17 @@
  * ({ VERSION, IC_INFO, createSolver, studySummary, Spectral, fftPlan, fft1, factorize }, nslab.js:668) and the same
  * solver object (nslab.js:648), with every transform, right-hand side, RK4 step and diagnostic computed in FP64 on the
  * GPU by libnslab_hip.so through the N-API addon. This file keeps the host-side bookkeeping the reference keeps on
- * the host: the mutable `st` object and its series (nslab.js:188-190, 300-302), the health report (nslab.js:595-635),
- * the dossier (nslab.js:637-645) and the refinement verdict (nslab.js:652-666).
+ * the host: the mutable `st` object and its series (nslab.js:188-190, 300-302); the health report, the dossier and
+ * the refinement verdict (nslab.js:595-666) are built by report.js from the numbers the GPU produced.
  *
  * There is no CPU path: if the addon cannot be loaded the require() throws.
  */
 'use strict';
 const path = require('path');
 const addon = require(path.join(__dirname, '..', 'lib', 'nslab_napi.node'));
+const report = require('./report.js');
 
 const VERSION = '0.1.2';
-const TWO_PI = 2 * Math.PI;
 const IC_KIND = { tgv: 0, tgv2d: 1, abc: 2, tubes: 3, random: 4, tubesz: 5 };
 
 const IC_INFO = {
@@ -136,7 +136,10 @@ function createSolver(opts, pool) {
       last = toDiag(r.subarray(o + 5, o + 13));
       record(st.dt); st.series.ebal.push(ebal); st.series.zbal.push(zbal);
     }
-    if (got > 0) devClock = { t: st.t, step: st.step, E0: st.E0 };
+    if (got > 0) {
+      devClock = { t: st.t, step: st.step, E0: st.E0 };
+      sOut = false;   // the device state moved on: the next _S read fetches it
+    }
   }
   /** run(nSteps, tEnd) (nslab.js:334): steps on the device in chunks; the series are booked as record() does. */
   function run(nSteps, tEnd) {
@@ -156,13 +159,8 @@ function createSolver(opts, pool) {
   function norms() { sync(); const a = addon.norms(h); return { E: a[0], Z: a[1], P: a[2] }; }
   function spectrum() { sync(); return Array.from(addon.spectrum(h)); }
 
-  const SPEC_FLOOR = 100 * Number.EPSILON * Number.EPSILON;
-  function cutoffPileUp(spec) {
-    const k8 = Math.round(0.8 * kc); let peak = 0; for (let k = 1; k < spec.length; k++) peak = Math.max(peak, spec[k]);
-    if (!(spec[k8] > SPEC_FLOOR * peak)) return null;
-    let m = 0; for (let k = k8; k <= kc; k++) m = Math.max(m, spec[k] / spec[k8]);
-    return m;
-  }
+  /** cutoffPileUp (nslab.js:557-564): the library's rule over a spectrum of this grid; null below the noise floor. */
+  function cutoffPileUp(spec) { return addon.cutoffPileUp(kc, Float64Array.from(spec)); }
 
   /** diagnose (nslab.js:339-392): the whole computation runs on the GPU; the result object has the reference's keys. */
   function diagnose() {
@@ -191,37 +189,19 @@ function createSolver(opts, pool) {
   const fields = {};
   for (const k of Object.keys(FIELD)) Object.defineProperty(fields, k, { enumerable: true, get: () => field(k) });
 
-  /** imageDiag (nslab.js:537-550), from the vorticity field. */
+  /** imageDiag (nslab.js:537-550): the enstrophy profile along z of the current vorticity, on the GPU. */
   function imageDiag() {
-    const ox = field('omx'), oy = field('omy'), oz = field('omz'), Ez = new Float64Array(N);
-    for (let l = 0; l < N; l++) { let s = 0; for (let n = l * N * N; n < (l + 1) * N * N; n++) s += ox[n] * ox[n] + oy[n] * oy[n] + oz[n] * oz[n]; Ez[l] = s; }
-    let mx = 0, cs = 0, sn = 0; for (let l = 0; l < N; l++) { mx = Math.max(mx, Ez[l]); const z = l * TWO_PI / N; cs += Ez[l] * Math.cos(z); sn += Ez[l] * Math.sin(z); }
-    const zc = ((Math.atan2(sn, cs) % TWO_PI) + TWO_PI) % TWO_PI;
-    const on = Array.from(Ez, v => v > 0.01 * mx); const runs = []; let start = null;
-    for (let l = 0; l < N; l++) { if (on[l] && start === null) start = l; if (!on[l] && start !== null) { runs.push([start, l]); start = null; } }
-    if (start !== null) runs.push([start, N]);
-    let rr = runs; if (runs.length > 1 && on[0] && on[N - 1]) rr = [[runs[runs.length - 1][0], runs[0][1] + N]].concat(runs.slice(1, -1));
-    const ext = Math.min((rr.length ? Math.max(...rr.map(r => r[1] - r[0])) : N) * TWO_PI / N, TWO_PI);
-    return { zCentroid: zc, zExtent: ext, imageGap: Math.max(TWO_PI - ext, 0), zBands: rr.length };
+    sync();
+    const a = addon.imageDiag(h);
+    return { zCentroid: a[0], zExtent: a[1], imageGap: a[2], zBands: a[3] };
   }
 
-  /** slice (nslab.js:566-583): N x N Float32 slice of a physical field. */
+  /** slice (nslab.js:566-583): N x N Float32 plane of a physical quantity, extracted on the GPU. */
+  const SLICE_KIND = { vort: 0, speed: 1, ke: 2, q: 3, stretch: 4, u: 5, v: 6, w: 7, omx: 8, omy: 9, omz: 10 };
   function slice(kind, axis, index) {
-    const out = new Float32Array(N * N);
-    let val;
-    if (kind === 'vort' || kind === 'speed' || kind === 'ke') {
-      const a = kind === 'vort' ? [field('omx'), field('omy'), field('omz')] : [field('u'), field('v'), field('w')];
-      val = kind === 'vort' ? n => Math.sqrt(a[0][n] * a[0][n] + a[1][n] * a[1][n] + a[2][n] * a[2][n])
-        : kind === 'speed' ? n => Math.sqrt(a[0][n] * a[0][n] + a[1][n] * a[1][n] + a[2][n] * a[2][n])
-          : n => 0.5 * (a[0][n] * a[0][n] + a[1][n] * a[1][n] + a[2][n] * a[2][n]);
-    } else if (kind in FIELD) { const f = field(kind); val = n => f[n]; }
-    else val = () => 0;
-    const ix = Math.max(0, Math.min(N - 1, index | 0));
-    for (let b = 0; b < N; b++) for (let a = 0; a < N; a++) {
-      const n = axis === 'z' ? (ix * N + b) * N + a : axis === 'y' ? (b * N + ix) * N + a : (b * N + a) * N + ix;
-      out[b * N + a] = val(n);
-    }
-    return out;
+    sync();
+    const k = kind in SLICE_KIND ? SLICE_KIND[kind] : -1;
+    return addon.slice(h, k, axis === 'z' ? 2 : axis === 'y' ? 1 : 0, index | 0);
   }
   function exactError() {
     sync();
@@ -229,61 +209,20 @@ function createSolver(opts, pool) {
     return a ? { linf: a[0], rel: a[1] } : null;
   }
 
-  /** Health report (nslab.js:595-635): the reference's rows, thresholds and formats. */
-  function health(study) {
-    const d = last, o = st.outputs.length ? st.outputs[st.outputs.length - 1] : null;
-    const eta = Math.pow(nu * nu * nu / Math.max(d.eps, 1e-300), 0.25), kmax = Math.sqrt(3) * kc, ke = kmax * eta;
-    const spec = o ? o.spectrum : spectrum(); let kp = 0; for (let k = 1; k < spec.length; k++) if (spec[k] > spec[kp]) kp = k;
-    const tail = spec[kc] / Math.max(spec[kp], 1e-300), pile = cutoffPileUp(spec);
-    const cflNow = d.uMax * st.dt / (TWO_PI / N);
-    const grade = (v, good, warn, lowerIsBetter = true) => (lowerIsBetter ? (v <= good ? 'PASS' : v <= warn ? 'WARN' : 'FAIL') : (v >= good ? 'PASS' : v >= warn ? 'WARN' : 'FAIL'));
-    const sdOf = q => Math.abs(q.Pspec - q.Pphys) / Math.max(Math.abs(q.Pspec), 2 * nu * q.P, 1e-300);
-    const rows = [
-      ['grid', `${N}³ (kmax ${kc}, dealiased 2/3)`, ''],
-      ['Δt', st.dt.toExponential(3) + (dtFixed ? ' fixed' : ' adaptive'), ''],
-      ['CFL (u_max Δt/Δx)', cflNow.toFixed(3), grade(cflNow, 0.8, 1.2)],
-      ['divergence L∞', st.divMax.toExponential(2), grade(st.divMax, 1e-10, 1e-6)],
-      ['nonlinear energy transfer |T|/ε', st.maxTnl.toExponential(2), grade(st.maxTnl, 1e-9, 1e-6)],
-      ['energy budget residual (RK4-consistent)', st.maxEbal.toExponential(2), grade(st.maxEbal, 1e-5, 1e-3)],
-      ['enstrophy budget residual', st.maxZbal.toExponential(2), grade(st.maxZbal, 1e-4, 1e-2)],
-      ['resolution kmax·η', ke.toFixed(2), grade(ke, 1.0, 0.5, false)],
-      ['spectral tail E(kmax)/E(peak)', tail.toExponential(2), grade(tail, 1e-4, 1e-2)],
-      ['stretching: spectral vs physical', o ? sdOf(o).toExponential(2) : '—', o ? grade(sdOf(o), 1e-2, 1e-1) : ''],
-      ['cutoff pile-up E(k)/E(0.8kmax)', pile != null ? pile.toFixed(2) : '—', pile != null ? grade(pile, 1.2, 2.0) : ''],
-      ['max |ω|', d.omMax.toFixed(4) + (o && o.omMaxI != null ? ` (interpolated ${o.omMaxI.toFixed(4)})` : ''), ''], ['enstrophy Z', d.Z.toFixed(5), ''], ['stretching ⟨ω·S·ω⟩', d.Pspec.toFixed(5), ''], ['dissipation ε = 2νZ', d.eps.toExponential(4), ''],
-      ['‖u‖_L³ (ESS criterion)', o && o.uL3 != null ? o.uL3.toFixed(5) : '—', ''],
-      ['periodic-image gap in z', o && o.imageGap != null ? o.imageGap.toFixed(2) : '—', ''],
-      ['grid convergence', study && study.grid ? study.grid.text : 'not run', study && study.grid ? study.grid.verdict : 'N/A'],
-      ['time-step convergence', study && study.time ? study.time.text : 'not run', study && study.time ? study.time.verdict : 'N/A'],
-    ];
-    const verdict = rs => (rs.some(r => r[2] === 'FAIL') ? 'FAIL' : rs.some(r => r[2] === 'WARN') ? 'WARN' : 'PASS');
-    const worstEnd = verdict(rows);
-    const snaps = st.outputs;
-    if (snaps.length) {
-      const tailOf = q => { let kq = 0; for (let k = 1; k < q.spectrum.length; k++) if (q.spectrum[k] > q.spectrum[kq]) kq = k; return q.spectrum[kc] / Math.max(q.spectrum[kq], 1e-300); };
-      const wKe = snaps.reduce((a, b) => (b.kmaxEta < a.kmaxEta ? b : a)), wT = snaps.reduce((a, b) => (tailOf(b) > tailOf(a) ? b : a)), wS = snaps.reduce((a, b) => (sdOf(b) > sdOf(a) ? b : a));
-      const wP = snaps.filter(q => q.pileUp != null).reduce((a, b) => ((a == null || b.pileUp > a.pileUp) ? b : a), null);
-      rows.push(['worst instant: kmax·η', `${wKe.kmaxEta.toFixed(2)} at t ${wKe.t.toFixed(2)}`, grade(wKe.kmaxEta, 1.0, 0.5, false)]);
-      rows.push(['worst instant: spectral tail', `${tailOf(wT).toExponential(2)} at t ${wT.t.toFixed(2)}`, grade(tailOf(wT), 1e-4, 1e-2)]);
-      rows.push(['worst instant: stretching spectral vs physical', `${sdOf(wS).toExponential(2)} at t ${wS.t.toFixed(2)}`, grade(sdOf(wS), 1e-2, 1e-1)]);
-      if (wP) rows.push(['worst instant: cutoff pile-up', `${wP.pileUp.toFixed(2)} at t ${wP.t.toFixed(2)}`, grade(wP.pileUp, 1.2, 2.0)]);
-    }
-    return { rows, worst: verdict(rows), worstEnd, note: 'A run that does not PASS resolution, budgets and refinement cannot support any claim about the equations. Numerical growth ≠ mathematical singularity. The verdict includes the worst archived snapshot; worstEnd is the end-of-run verdict alone.' };
-  }
+  // what report.js reads of this solve
+  const run_ = { N, kc, nu, Re, cfl, dtFixed, st, get last() { return last; }, spectrum, pileUp: cutoffPileUp };
+  /** Health report (nslab.js:595-635). */
+  function health(study) { return report.healthReport(run_, study); }
   /** Evidence dossier (nslab.js:637-645). */
   function dossier(extra) {
-    return Object.assign({
-      instrument: 'Pocket Wind Tunnel · NSLab ' + VERSION + ' (MI355X FP64, ' + addon.version() + ')', build: null,
-      equations: '3D incompressible Navier–Stokes, periodic [0,2π]³, Re = 1/ν, Fourier pseudo-spectral (2/3 dealiased, rotational form), RK4',
-      parameters: { N, Re, nu, cfl, dt: dtFixed || 'adaptive', ic: st.ic, icParams: st.icParams, kc },
-      state: { t: st.t, steps: st.step }, series: st.series, outputs: st.outputs, health: health(extra && extra.study),
-      disclaimer: 'Numerical evidence only. Nothing here is a proof of global regularity or of finite-time blow-up of the Navier–Stokes equations; every feature must survive spatial and temporal refinement before it is even a conjecture.',
-    }, extra || {});
+    return report.dossierOf(run_, 'Pocket Wind Tunnel · NSLab ' + VERSION + ' (MI355X FP64, ' + addon.version() + ')', extra);
   }
+  /** Frees the device memory now (a pool member is left to its pool); later calls on this solver throw. */
+  function destroy() { if (!pool) addon.destroy(h); }
 
   finishIC();
   const solver = { st, get last() { return last; }, step, run, diagnose, spectrum, slice, exactError, health, dossier, setIC,
-    norms, interpMax, imageDiag, fields, N, nu, Re, kc, _sp: sp,
+    norms, interpMax, imageDiag, fields, N, nu, Re, kc, _sp: sp, destroy,
     deviceBytes: () => addon.deviceBytes(h),
     profile: on => addon.profile(h, on ? 1 : 0), profileRead: () => addon.profileRead(h),
     _slot: slot, _book: book, _sync: sync };
@@ -298,21 +237,7 @@ function createSolver(opts, pool) {
 }
 
 /** Refinement study verdict (nslab.js:652-666). */
-function studySummary(levels, key) {
-  if (levels.length < 2) return null;
-  const rel = (a, b) => Math.abs(a - b) / Math.max(Math.abs(b), 1e-300);
-  const rows = []; let lastChange = null, prevChange = null;
-  for (let i = 1; i < levels.length; i++) {
-    const a = levels[i - 1], b = levels[i];
-    const c = { from: a.label, to: b.label, omMax: rel(a.peak.omMax, b.peak.omMax), eps: rel(a.epsPeak.v, b.epsPeak.v), Eend: rel(a.Eend, b.Eend), tPeak: Math.abs(a.epsPeak.t - b.epsPeak.t) };
-    rows.push(c); prevChange = lastChange; lastChange = Math.max(c.omMax, c.eps, c.Eend);
-  }
-  const order = prevChange != null && lastChange > 0 ? Math.log(prevChange / lastChange) / Math.log(key === 'time' ? 2 : levels[levels.length - 1].N / levels[levels.length - 2].N) : null;
-  const verdict = lastChange < 0.01 && (prevChange == null || lastChange < prevChange) ? 'PASS' : lastChange < 0.05 ? 'WARN' : 'FAIL';
-  const last = rows[rows.length - 1];
-  const text = `${levels.map(l => l.label).join(' → ')}: last change ω_max ${(last.omMax * 100).toFixed(2)} %, ε_peak ${(last.eps * 100).toFixed(2)} %, E(t_end) ${(last.Eend * 100).toFixed(3)} %` + (order != null ? `, observed order ${order.toFixed(2)}` : '');
-  return { rows, verdict, text, order };
-}
+const studySummary = report.studySummary;
 
 // ------------------------------------------------------------------------------------------------ pools
 /**
@@ -338,7 +263,11 @@ class Pool {
   /** The solver's next launches run up to n steps, stopping at tEnd (run(n, tEnd) semantics). */
   setTarget(s, n, tEnd) { s._sync(); addon.poolSetTarget(this.h, s._slot, n, tEnd != null ? 1 : 0, tEnd != null ? tEnd : 0); }
   /** Enqueue up to k (<= 64) steps of every slot with steps left; returns at once. */
-  launch(k) { this.k = k || 64; addon.poolLaunch(this.h, this.k); }
+  launch(k) {
+    for (const s of this.members.values()) s._sync();
+    this.k = k || 64;
+    addon.poolLaunch(this.h, this.k);
+  }
   /** True once the launched steps have finished (collect() will not wait). */
   ready() { return addon.poolReady(this.h); }
   /** Wait for the launched steps, book them into each member's series; returns the number of slots still running. */
@@ -348,6 +277,8 @@ class Pool {
     return r[0];
   }
   release(s) { addon.poolRelease(this.h, s._slot); this.members.delete(s._slot); }
+  /** Frees the pool's device memory now; its members must not be used afterwards. */
+  destroy() { addon.poolDestroy(this.h); this.members.clear(); }
 }
 
 function deviceMemory() { const a = addon.deviceMemory(); return { free: a[0], total: a[1] }; }

@@ -101,6 +101,7 @@ function runEnsemble(opts = {}) {
   // sequential single solves on the same GPU: `seqPerN` solves of each N, run one at a time through createSolver,
   // extrapolated by the mix counts; the first solve of each N is also checked bit for bit against its pool slot
   const seqPerN = opts.seqPerN != null ? opts.seqPerN : 8;
+  for (const p of pools) p.pool.destroy();   // the series are on the host; free the device for the sequential leg
   if (seqPerN > 0) {
     let tSeq = 0, identical = true;
     const perN = [];
@@ -115,6 +116,7 @@ function runEnsemble(opts = {}) {
           identical = identical && a.E.length === b.E.length && a.E.every((v, i) => v === b.E[i]) &&
             a.omMax.every((v, i) => v === b.omMax[i]) && s.st.t === m.solver.st.t;
         }
+        s.destroy();
       }
       const per = (performance.now() - t0) / 1e3 / take.length;
       perN.push({ N: p.N, sampled: take.length, s_per_solve: per });
@@ -273,10 +275,12 @@ function serve(opts = {}) {
   }
   const wall = now(), lat = results.map(r => r.latency), rungSteps = results.reduce((a, r) => a + r.levels.reduce((b, l) => b + l.steps, 0), 0);
   const solves = results.reduce((a, r) => a + r.levels.length, 0), sweeps = results.filter(r => r.kind === 'sweep').length;
+  const poolList = pools.map(p => ({ N: p.N, slots: p.pool.slots }));
+  for (const p of pools) p.pool.destroy();
   return { requests: reqs.length, sweeps, rate: opts.rate || 2, wall_s: wall, requests_per_hour: reqs.length / wall * 3600,
     ladders_per_hour: (reqs.length - sweeps) / wall * 3600, sweeps_per_hour: sweeps / wall * 3600,
     rung_solves_per_hour: solves / wall * 3600, p50_latency_s: quantile(lat, 0.5), p99_latency_s: quantile(lat, 0.99),
-    rung_steps: rungSteps, pools: pools.map(p => ({ N: p.N, slots: p.pool.slots })), reserved_bytes: reserved,
+    rung_steps: rungSteps, pools: poolList, reserved_bytes: reserved,
     budget_bytes: budget, results };
 }
 
@@ -290,6 +294,7 @@ function serveSequential(reqs, results, sample) {
       const s = NS.createSolver({ N: m.N, Re: m.Re, cfl: 0.5, ic: q.ic, icParams: q.icParams });
       s.run(20 * q.steps, R.tEnd);
       levels.push({ N: m.N, Re: m.Re, steps: s.st.step, t: s.st.t, Eend: s.st.series.E[s.st.series.E.length - 1] });
+      s.destroy();
     }
     out.push({ id: q.id, levels });
   }
@@ -309,6 +314,8 @@ function singleSolveCosts(Ns, steps = 60) {
     s.run(steps);
     const t2 = performance.now();
     out[N] = { create_s: (t1 - t0) / 1e3, step_s: (t2 - t1) / 1e3 / steps };
+    w.destroy();
+    s.destroy();
   }
   return out;
 }

@@ -225,13 +225,18 @@ static int planes_of(napi_env env, napi_value arr, double *planes[6], size_t wan
   return 1;
 }
 
+/* Complex modes per half-spectrum plane of the solver's grid, N * N * (N/2 + 1). */
+static size_t plane_len(nsl_solver *s) {
+  const size_t N = (size_t)nsl_grid_size(s);
+  return N * N * (N / 2 + 1);
+}
+
 static napi_value js_get_state(napi_env env, napi_callback_info info) {
   ARGS(3);
   SOLVER(0);
-  int64_t NS = 0;
-  get_i64(env, argv[2], &NS);
+  const size_t NS = plane_len(s);
   double *pl[6];
-  if (!planes_of(env, argv[1], pl, (size_t)NS)) { napi_throw_type_error(env, NULL, "state needs 6 Float64Array of NS"); return NULL; }
+  if (!planes_of(env, argv[1], pl, NS)) { napi_throw_range_error(env, NULL, "state needs 6 Float64Array of N*N*(N/2+1)"); return NULL; }
   int rc = nsl_get_state(s, pl);
   if (rc < 0) return throw_status(env, rc);
   return NULL;
@@ -240,10 +245,9 @@ static napi_value js_get_state(napi_env env, napi_callback_info info) {
 static napi_value js_set_state(napi_env env, napi_callback_info info) {
   ARGS(3);
   SOLVER(0);
-  int64_t NS = 0;
-  get_i64(env, argv[2], &NS);
+  const size_t NS = plane_len(s);
   double *pl[6];
-  if (!planes_of(env, argv[1], pl, (size_t)NS)) { napi_throw_type_error(env, NULL, "state needs 6 Float64Array of NS"); return NULL; }
+  if (!planes_of(env, argv[1], pl, NS)) { napi_throw_range_error(env, NULL, "state needs 6 Float64Array of N*N*(N/2+1)"); return NULL; }
   int rc = nsl_set_state(s, (const double *const *)pl);
   if (rc < 0) return throw_status(env, rc);
   return NULL;
@@ -297,6 +301,8 @@ static napi_value js_field(napi_env env, napi_callback_info info) {
   double *p;
   size_t n;
   if (!get_f64arr(env, argv[2], &p, &n)) { napi_throw_type_error(env, NULL, "field needs a Float64Array"); return NULL; }
+  const size_t N = (size_t)nsl_grid_size(s);
+  if (n < N * N * N) { napi_throw_range_error(env, NULL, "field needs a Float64Array of N^3"); return NULL; }
   int rc = nsl_field(s, (int32_t)which, p);
   if (rc < 0) return throw_status(env, rc);
   return NULL;
@@ -312,6 +318,54 @@ static napi_value js_exact_error(napi_env env, napi_callback_info info) {
   double *p;
   napi_value out = new_f64arr(env, 2, &p);
   p[0] = e[0]; p[1] = e[1];
+  return out;
+}
+
+/* slice(h, kind, axis, index) -> Float32Array(N*N) computed on the GPU (kind, axis as nsl_slice) */
+static napi_value js_slice(napi_env env, napi_callback_info info) {
+  ARGS(4);
+  SOLVER(0);
+  int64_t kind = -1, axis = 0, index = 0;
+  get_i64(env, argv[1], &kind);
+  get_i64(env, argv[2], &axis);
+  get_i64(env, argv[3], &index);
+  const size_t N = (size_t)nsl_grid_size(s);
+  napi_value ab, ta;
+  void *p;
+  CALL(env, napi_create_arraybuffer(env, N * N * sizeof(float), &p, &ab));
+  CALL(env, napi_create_typedarray(env, napi_float32_array, N * N, ab, 0, &ta));
+  int rc = nsl_slice(s, (int32_t)kind, (int32_t)axis, (int32_t)index, (float *)p);
+  if (rc < 0) return throw_status(env, rc);
+  return ta;
+}
+
+/* imageDiag(h) -> [zCentroid, zExtent, imageGap, zBands] */
+static napi_value js_image_diag(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  double *p;
+  napi_value out = new_f64arr(env, 4, &p);
+  int rc = nsl_image_diag(s, p);
+  if (rc < 0) return throw_status(env, rc);
+  return out;
+}
+
+/* cutoffPileUp(kc, Float64Array spectrum) -> number, or null below the noise floor */
+static napi_value js_cutoff_pile_up(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  int64_t kc = 0;
+  double *sp;
+  size_t nb;
+  if (!get_i64(env, argv[0], &kc) || !get_f64arr(env, argv[1], &sp, &nb)) {
+    napi_throw_type_error(env, NULL, "cutoffPileUp(kc, Float64Array spectrum)");
+    return NULL;
+  }
+  double v = 0;
+  int rc = nsl_cutoff_pile_up((int32_t)kc, sp, (int32_t)nb, &v);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value out;
+  if (rc == 0) CALL(env, napi_get_null(env, &out));
+  else CALL(env, napi_create_double(env, v, &out));
   return out;
 }
 
@@ -340,6 +394,14 @@ static napi_value js_fft1(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* The library reads or writes N^3 reals and N*N*(N/2+1) modes whatever the arrays hold: refuse shorter ones (a grid
+ * the library rejects is left to its own error). */
+static int spectral_sizes_ok(int64_t N, size_t nreal, size_t nre, size_t nim) {
+  if (N < 4 || N > 4096) return 1;
+  const size_t n = (size_t)N, ns = n * n * (n / 2 + 1);
+  return nreal >= n * n * n && nre >= ns && nim >= ns;
+}
+
 static napi_value js_spectral_forward(napi_env env, napi_callback_info info) {
   ARGS(4);
   int64_t N = 0;
@@ -348,6 +410,10 @@ static napi_value js_spectral_forward(napi_env env, napi_callback_info info) {
   get_i64(env, argv[0], &N);
   if (!get_f64arr(env, argv[1], &f, &n1) || !get_f64arr(env, argv[2], &re, &n2) || !get_f64arr(env, argv[3], &im, &n3)) {
     napi_throw_type_error(env, NULL, "spectralForward(N, f, re, im)");
+    return NULL;
+  }
+  if (!spectral_sizes_ok(N, n1, n2, n3)) {
+    napi_throw_range_error(env, NULL, "spectralForward needs f of N^3 and re, im of N*N*(N/2+1) values");
     return NULL;
   }
   int rc = nsl_spectral_forward((int32_t)N, f, re, im);
@@ -363,6 +429,10 @@ static napi_value js_spectral_inverse(napi_env env, napi_callback_info info) {
   get_i64(env, argv[0], &N);
   if (!get_f64arr(env, argv[1], &re, &n1) || !get_f64arr(env, argv[2], &im, &n2) || !get_f64arr(env, argv[3], &f, &n3)) {
     napi_throw_type_error(env, NULL, "spectralInverse(N, re, im, f)");
+    return NULL;
+  }
+  if (!spectral_sizes_ok(N, n3, n1, n2)) {
+    napi_throw_range_error(env, NULL, "spectralInverse needs re, im of N*N*(N/2+1) and f of N^3 values");
     return NULL;
   }
   int rc = nsl_spectral_inverse((int32_t)N, re, im, f);
@@ -699,6 +769,7 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"poolDestroy", js_pool_destroy},
     {"deviceMemory", js_device_memory}, {"poolReady", js_pool_ready},
     {"tuneGet", js_tune_get}, {"tuneSet", js_tune_set}, {"tuneCandidates", js_tune_candidates},
+    {"slice", js_slice}, {"imageDiag", js_image_diag}, {"cutoffPileUp", js_cutoff_pile_up},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

@@ -116,6 +116,33 @@ int nsl_field(nsl_solver *s, int32_t which, double *out);
 /* solver.exactError() (nslab.js:585-594): returns 1 and fills {linf, rel} for abc/tgv2d, 0 for other ICs. */
 int nsl_exact_error(nsl_solver *s, double out[2]);
 
+/* solver.slice(kind, axis, index) (nslab.js:566-583): an N x N float32 plane of a physical quantity, out[b*N + a],
+ * computed on the GPU. axis 2 is the plane z = index (a = x, b = y), 1 is y = index (a = x, b = z), anything else
+ * x = index (a = y, b = z); index is clamped to 0..N-1. |omega|, |u|, u.u/2, u, v, w and omega are those of the
+ * current state; Q and the stretching density those of the last nsl_diagnose (zero before one). Other kinds give a
+ * plane of zeros, as the reference's slice does for an unknown kind. */
+#define NSL_SLICE_VORT 0
+#define NSL_SLICE_SPEED 1
+#define NSL_SLICE_KE 2
+#define NSL_SLICE_Q 3
+#define NSL_SLICE_STRETCH 4
+#define NSL_SLICE_U 5
+#define NSL_SLICE_V 6
+#define NSL_SLICE_W 7
+#define NSL_SLICE_OMX 8
+#define NSL_SLICE_OMY 9
+#define NSL_SLICE_OMZ 10
+int nsl_slice(nsl_solver *s, int32_t kind, int32_t axis, int32_t index, float *out);
+/* solver.imageDiag() (nslab.js:537-550) of the current state's vorticity: out = {zCentroid, zExtent, imageGap,
+ * zBands}. */
+int nsl_image_diag(nsl_solver *s, double out[4]);
+/* The cutoff pile-up of a shell spectrum (cutoffPileUp, nslab.js:557-564) on a grid with kc = floor(N/3): the largest
+ * E(k)/E(k8) for k8 = round(0.8 kc) <= k <= kc. Returns 1 and *out, or 0 (the reference's null) when E(k8) is not
+ * above 100 eps^2 times the largest E(k), k >= 1; NSL_ERR_ARG unless 1 <= kc < nb. Host arithmetic only. */
+int nsl_cutoff_pile_up(int32_t kc, const double *spec, int32_t nb, double *out);
+/* solver.N: the grid size of a solver or pool slot. */
+int nsl_grid_size(const nsl_solver *s);
+
 /* Bytes of device memory the solver holds (admission accounting). */
 int64_t nsl_device_bytes(const nsl_solver *s);
 
@@ -150,7 +177,10 @@ int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count);
  *   nsl_pool_ready(p)              1 once the launched steps have finished (collect will not wait), else 0
  *   nsl_pool_collect(p, recs, cap, n_done, &running)   waits; per slot i: n_done[i] steps and their records at
  *                                  recs[i*cap ..] (recs may be null); running = slots with steps left
- *   nsl_pool_release(p, i)         frees the slot */
+ *   nsl_pool_release(p, i)         frees the slot
+ * Between nsl_pool_launch and nsl_pool_collect a slot that took part in the launched chunk belongs to it:
+ * nsl_pool_set_target, nsl_run, nsl_set_state and nsl_set_ic on it return NSL_ERR_STATE. Slots admitted or
+ * re-targeted after the launch were not part of it and may be set up while it runs. */
 typedef struct nsl_pool nsl_pool;
 int64_t nsl_pool_slot_bytes(int32_t N);
 /* Free and total device memory of the current GPU (the admission budget of a request loop). */
@@ -195,6 +225,11 @@ int nsl_tune_get(int32_t N, nsl_tune *out);
 int nsl_tune_set(int32_t N, const nsl_tune *cfg);
 /* Every compiled form of N into out[0..min(return, cap)); returns the count. */
 int nsl_tune_candidates(int32_t N, nsl_tune *out, int32_t cap);
+/* The x pass's launch plan for form cfg at grid N on a device of `cus` compute units (0: the current device's):
+ * *workgroups receives how many workgroups it launches, each leaving one maxima pair. NSL_ERR_STATE when the form
+ * is not compiled for N or when those pairs exceed cap_pairs. A solver checks this plan against its buffer of N^2
+ * pairs before it enqueues the pass, so an oversized grid is refused with nothing launched. Host arithmetic only. */
+int nsl_xpass_plan(int32_t N, const nsl_tune *cfg, int32_t cus, int64_t cap_pairs, int64_t *workgroups);
 
 /* Text of the last error on this thread ("" if none). */
 const char *nsl_last_error(void);

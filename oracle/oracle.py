@@ -60,6 +60,8 @@ def lib():
         L.or_spectral_forward.argtypes = [ctypes.c_int, dp, dp, dp]
         L.or_spectral_inverse.restype = ctypes.c_int
         L.or_spectral_inverse.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.or_cutoff_pile_up.restype = ctypes.c_int
+        L.or_cutoff_pile_up.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -102,6 +104,13 @@ def fft1(xr, xi, sign):
     if lib().or_fft1(len(re), int(sign), _p(re), _p(im)) != 0:
         raise ValueError("bad N")
     return re, im
+
+
+def cutoff_pile_up(kc, spec):
+    """cutoffPileUp (nslab.js:557-564) of a shell spectrum on a grid with kc = floor(N/3); None below the floor."""
+    sp = np.ascontiguousarray(spec, dtype=np.float64)
+    out = np.zeros(1)
+    return float(out[0]) if lib().or_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out)) else None
 
 
 def spectral_forward(f):

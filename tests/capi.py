@@ -97,6 +97,11 @@ def lib():
         L.nsl_tune_get.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
         L.nsl_tune_set.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
         L.nsl_tune_candidates.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32]
+        L.nsl_slice.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
+        L.nsl_image_diag.argtypes = [vp, dp]
+        L.nsl_cutoff_pile_up.argtypes = [ctypes.c_int32, dp, ctypes.c_int32, dp]
+        L.nsl_grid_size.argtypes = [vp]
+        L.nsl_xpass_plan.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32, ctypes.c_int64, i64p]
         L.nsl_last_error.restype = ctypes.c_char_p
         L.nsl_version.restype = ctypes.c_char_p
         _lib = L
@@ -285,6 +290,21 @@ class Solver:
             return None
         return dict(linf=o[0], rel=o[1])
 
+    SLICE = {"vort": 0, "speed": 1, "ke": 2, "q": 3, "stretch": 4, "u": 5, "v": 6, "w": 7, "omx": 8, "omy": 9,
+             "omz": 10}
+
+    def slice(self, kind, axis, index):
+        out = np.zeros(self.N * self.N, dtype=np.float32)
+        k = self.SLICE.get(kind, -1)
+        a = 2 if axis == "z" else 1 if axis == "y" else 0
+        check(lib().nsl_slice(self.h, k, a, int(index), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def image_diag(self):
+        o = np.zeros(4)
+        check(lib().nsl_image_diag(self.h, _p(o)))
+        return {"zCentroid": o[0], "zExtent": o[1], "imageGap": o[2], "zBands": o[3]}
+
     def device_bytes(self):
         return lib().nsl_device_bytes(self.h)
 
@@ -311,6 +331,22 @@ def spectral_inverse(N, re, im):
     f = np.zeros(N ** 3)
     check(lib().nsl_spectral_inverse(N, _p(re), _p(im), _p(f)))
     return f
+
+
+def cutoff_pile_up(kc, spec):
+    """The library's cutoff pile-up of a shell spectrum (host arithmetic; no GPU needed). None below the floor."""
+    sp = np.ascontiguousarray(spec, dtype=np.float64)
+    out = np.zeros(1)
+    return float(out[0]) if check(lib().nsl_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out))) == 1 else None
+
+
+def xpass_plan(N, cfg, cus=256, cap_pairs=None):
+    """Workgroups the x pass launches for form cfg at grid N (its maxima pairs checked against cap_pairs)."""
+    t = Tune(*cfg)
+    g = ctypes.c_int64(0)
+    check(lib().nsl_xpass_plan(int(N), ctypes.byref(t), int(cus), int(N * N if cap_pairs is None else cap_pairs),
+                               ctypes.byref(g)))
+    return g.value
 
 
 def tune_get(N):

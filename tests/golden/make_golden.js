@@ -1,7 +1,7 @@
 // Writes the golden fixtures under tests/golden/ by running the NSLab reference solver under Node.
 //
 // Usage (only where the reference checkout is present; the fixtures it writes are committed):
-//   node tests/golden/make_golden.js /root/reference/pocket-wind-tunnel/src/nslab.js [small|long]
+//   node tests/golden/make_golden.js /root/reference/pocket-wind-tunnel/src/nslab.js [small|long|seeded [sizes]]
 //
 // What is written, and why no closed form or archived file pins it instead:
 //   * full spectral states after 0/100/200 fixed-dt RK4 steps (TGV 32^3, nu = 1/1600, dt = 0.01): the archive holds
@@ -161,6 +161,27 @@ if (mode === 'small') {
       levels.push({ label: `${N}³`, N, dt: 0, peak: { omMax: s.omMax[io], t: s.t[io] }, epsPeak: { v: s.eps[ie], t: s.t[ie] }, Eend: s.E[s.E.length - 1], steps: q.st.step });
     }
     writeJSON('ladder_re100.json', { levels, summary: NS.studySummary(levels, 'grid') });
+  }
+} else if (mode === 'seeded') {
+  // Seeded random solenoidal fields (k0 4, E0 0.5, Re 1000, CFL 0.5) stepped 100 times at every grid the ensemble and
+  // the ladders use below 192^3: the size-specialised kernels of 32..128 against the reference on a field with no
+  // symmetry. 32^3 and 48^3 keep the whole state; 64..128 keep every `stride`-th compact mode (about 2000) and the
+  // series.
+  const sizes = (process.argv[4] || '32,48,64,96,128').split(',').map(Number);
+  for (const N of sizes) {
+    const seed = 100 + N, t0 = Date.now();
+    const s = NS.createSolver({ N, Re: 1000, cfl: 0.5, ic: 'random', icParams: { seed, k0: 4, E0: 0.5 } });
+    const c0 = compactState(N, s._S);
+    s.run(100);
+    const c1 = compactState(N, s._S);
+    const nm = c1.length / 6, stride = N <= 48 ? 1 : Math.floor(nm / 2000);
+    const pick = c => { const o = new Float64Array(6 * Math.ceil(nm / stride)); let q = 0;
+      for (let p = 0; p < 6; p++) for (let m = 0; m < nm; m += stride) o[q++] = c[p * nm + m]; return o; };
+    writeF64(`random${N}_s${seed}_s000.f64.gz`, pick(c0));
+    writeF64(`random${N}_s${seed}_s100.f64.gz`, pick(c1));
+    writeJSON(`random${N}_s${seed}.json`, { opts: { N, Re: 1000, cfl: 0.5, ic: 'random', icParams: { seed, k0: 4, E0: 0.5 } },
+      stride, modes: nm, st: stScalars(s), last: s.last, series: s.st.series, secs: (Date.now() - t0) / 1000 });
+    console.log(`random ${N}^3 seed ${seed}: 100 steps in ${(Date.now() - t0) / 1000} s`);
   }
 } else if (mode === 'long') {
   // 64^3 Taylor-Green, nu = 1/1600, fixed dt = 0.004 to t = 10

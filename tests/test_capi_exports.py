@@ -84,3 +84,26 @@ def test_tune_table_and_overrides_without_gpu():
     assert capi.tune_get(64) == (1, 8, 0, 0)
     capi.tune_set(64)
     assert capi.tune_get(64) == table[64]
+
+
+def test_xpass_plan_is_checked_before_the_launch():
+    """The x pass's workgroups each leave one maxima pair in a buffer of N^2 pairs per slot. The solver computes the
+    launch plan and checks it against that buffer before enqueuing the pass (nsl_xpass_plan is the same function):
+    every compiled form of every size fits on 256 and 1024 CUs, and a buffer one pair short is refused."""
+    for N in (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
+        for cfg in capi.tune_candidates(N):
+            for cus in (256, 1024):
+                g = capi.xpass_plan(N, cfg, cus=cus)
+                assert 1 <= g <= N * N, (N, cfg, cus, g)
+                try:
+                    capi.xpass_plan(N, cfg, cus=cus, cap_pairs=g - 1)
+                    raise AssertionError("expected a refusal for %s at %d" % (cfg, N))
+                except capi.NslError as e:
+                    assert "exceed their buffer" in str(e)
+    # the staged form at 1024^3 without persistence: two waves per row, one pair per workgroup, N^2 workgroups
+    assert capi.xpass_plan(1024, (4, 0, 0, 1)) == 1024 * 1024
+    try:
+        capi.xpass_plan(96, (3, 0, 0, 0))
+        raise AssertionError("expected an error")
+    except capi.NslError as e:
+        assert "not compiled" in str(e)

@@ -31,6 +31,15 @@ def test_module_loads_and_exports_reference_surface():
 
 
 @needs_node
+def test_report_module_reproduces_reference_reports():
+    """No GPU needed: js/report.js rebuilds health reports, verdicts and a ladder summary the reference wrote, and the
+    host's pile-up rule reproduces the archived FP64 values (tests/js/validate_report.js)."""
+    r = subprocess.run([node, os.path.join(ROOT, "tests", "js", "validate_report.js")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout + r.stderr
+
+
+@needs_node
 @pytest.mark.gpu
 @pytest.mark.parametrize("group", ["fft", "exact", "tgv", "misc", "config1"])
 def test_validate_nslab_on_gpu(group):
