@@ -219,15 +219,20 @@ __global__ void k_slice(const double *__restrict__ ph, int N, int kind, int axis
     const size_t n = axis == 2 ? ((size_t)ix * N + b) * N + a : axis == 1 ? ((size_t)b * N + ix) * N + a
                                                                          : ((size_t)b * N + a) * N + ix;
     double v = 0.0;
-    if (kind <= 2) {   // magnitudes: of omega (kind 0) or u (1, 2)
-      const double *f = ph + (kind == 0 ? PH_OM : PH_U) * N3;
-      const double x = f[n], y = f[N3 + n], z = f[2 * N3 + n];
-      const double sq = x * x + y * y + z * z;
-      v = kind == 2 ? 0.5 * sq : sqrt(sq);
-    } else if (kind == 3) v = ph[PH_Q * N3 + n];
-    else if (kind == 4) v = ph[PH_PST * N3 + n];
-    else if (kind <= 7) v = ph[(size_t)(PH_U + kind - 5) * N3 + n];
-    else if (kind <= 10) v = ph[(size_t)(PH_OM + kind - 8) * N3 + n];
+    switch (kind) {
+      case 0: case 1: case 2: {   // magnitudes: of omega (kind 0) or u (1, 2)
+        const double *f = ph + (kind == 0 ? PH_OM : PH_U) * N3;
+        const double x = f[n], y = f[N3 + n], z = f[2 * N3 + n];
+        const double sq = x * x + y * y + z * z;
+        v = kind == 2 ? 0.5 * sq : sqrt(sq);
+        break;
+      }
+      case 3: v = ph[PH_Q * N3 + n]; break;
+      case 4: v = ph[PH_PST * N3 + n]; break;
+      case 5: case 6: case 7: v = ph[(size_t)(PH_U + kind - 5) * N3 + n]; break;
+      case 8: case 9: case 10: v = ph[(size_t)(PH_OM + kind - 8) * N3 + n]; break;
+      default: break;             // the host never launches other kinds
+    }
     out[q] = (float)v;
   }
 }
@@ -651,8 +656,12 @@ static int field_impl(nsl_solver *s, int which, double *out) {
 
 static int slice_impl(nsl_solver *s, int kind, int axis, int index, float *out) {
   const int N = s->N;
+  if (kind < NSL_SLICE_VORT || kind > NSL_SLICE_OMZ) {   // an unknown kind: a plane of zeros, nothing launched
+    memset(out, 0, (size_t)N * N * sizeof(float));
+    return NSL_OK;
+  }
   TRY(ensure_phys(s));
-  if (!s->diag_valid && (kind <= 2 || (kind >= 5 && kind <= 10))) TRY(physical_u_om(s));
+  if (!s->diag_valid && kind != NSL_SLICE_Q && kind != NSL_SLICE_STRETCH) TRY(physical_u_om(s));
   const int ix = std::max(0, std::min(N - 1, index));
   TmpBuf d;
   HIP_TRY(hipMalloc(&d.p, (size_t)N * N * sizeof(float)));

@@ -3,7 +3,15 @@ against the reference itself on a field with no symmetry: a seeded random soleno
 adaptive CFL 0.5, stepped 100 times by nslab.js (tests/golden/random{N}_s{seed}*, written by make_golden.js seeded).
 The initial realisation and the state after 100 steps must agree to 1e-13 relative (L-infinity over the stored
 modes: all retained modes at 32^3 and 48^3, every `stride`-th one, about 2000, at 64^3 and up), and so must the
-per-step series and the step clock."""
+step clock and the series of t, dt, max|omega| and max|u|.
+
+Series that are sums over the modes (E, Z, eps, P) meet a limit of the reference's own: nslab.js sums them
+sequentially over all N^2 (N/2 + 1) entries, which is off from the exactly rounded sum by up to 2.4e-13 relative at
+96^3 (tests/test_oracle.py measures it on the oracle, which follows the same order). They are compared with the
+reference at 1e-12, and the GPU's own sums (fixed-order trees) are compared with an exactly rounded sum (math.fsum)
+of the GPU state at 4e-15 (a fixed-order tree over 10^5..10^6 terms is within about log2(n) eps)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -14,7 +22,8 @@ from conftest import rel_inf
 pytestmark = pytest.mark.gpu
 
 SIZES = [(32, 132), (48, 148), (64, 164), (96, 196), (128, 228)]
-SERIES = ["t", "dt", "E", "Z", "eps", "omMax", "uMax", "Pal"]
+SERIES_EXACT = ["t", "dt", "omMax", "uMax"]      # maxima and the clock: no summation
+SERIES_SUMS = ["E", "Z", "eps", "Pal"]          # sums over the modes
 
 
 def sampled(N, S, stride):
@@ -44,11 +53,25 @@ def test_seeded_random_field_100_steps_matches_reference(N, seed):
     err = rel_inf(a1, b1)
     print("  state after 100 steps: rel err %.2e" % err, flush=True)
     assert err < 1e-13, N
-    for k in SERIES:
+    for k in SERIES_EXACT:
         assert rel_inf(s.series[k], g["series"][k]) < 1e-13, (N, k)
+    for k in SERIES_SUMS:
+        assert rel_inf(s.series[k], g["series"][k]) < 1e-12, (N, k)
     c = s.clock
     assert c["step"] == g["st"]["step"]
     assert abs(c["t"] - g["st"]["t"]) <= 1e-13 * g["st"]["t"]
-    for k in ("E", "Z", "P", "eps", "omMax", "uMax"):
+    for k in ("omMax", "uMax"):
         assert abs(s.last[k] - g["last"][k]) <= 1e-13 * abs(g["last"][k]), (N, k)
+    for k in ("E", "Z", "P", "eps"):
+        assert abs(s.last[k] - g["last"][k]) <= 1e-12 * abs(g["last"][k]), (N, k)
+    # the GPU's sums against exactly rounded sums of its own state
+    NH = N // 2 + 1
+    S = s.state().reshape(6, N, N, NH)
+    kw = np.fft.fftfreq(N, 1.0 / N)
+    k2 = (np.arange(NH)[None, None, :] ** 2 + kw[None, :, None] ** 2 + kw[:, None, None] ** 2)
+    hw = np.where((np.arange(NH) == 0) | (np.arange(NH) == N // 2), 1.0, 2.0)[None, None, :]
+    e = 0.5 * hw * sum(S[c] ** 2 for c in range(6))
+    for k, w in (("E", 1.0), ("Z", k2), ("P", k2 * k2)):
+        exact = math.fsum((w * e).ravel())
+        assert abs(s.last[k] - exact) <= 4e-15 * exact, (N, k, s.last[k] - exact)
     s.close()
