@@ -346,19 +346,32 @@ static unsigned diag_blocks(size_t N3) {
   return (unsigned)std::min<size_t>(std::max<size_t>((N3 + NSL_NT - 1) / NSL_NT, 1), 2048);
 }
 
-static int spectrum_dev(nsl_solver *s, double *out) {
+// The shell spectrum's launch shape: nblk workgroups each leave nb bin partials, summed in a fixed order into nb
+// totals. part and tot are scratch of spectrum_scratch_doubles(s) doubles (partials first, then totals).
+static unsigned spectrum_blocks(const nsl_solver *s) {
+  return (unsigned)std::min<long long>(std::max<long long>((s->x.g.D + 4095) / 4096, 1), 1024);
+}
+static size_t spectrum_scratch_doubles(nsl_solver *s) {
+  return ((size_t)spectrum_blocks(s) + 1) * (size_t)nsl_spectrum_len(s);
+}
+// Every workgroup writes all nb of its partials and the sum writes all nb totals: nothing needs clearing first.
+static int spectrum_run(nsl_solver *s, double *part, double *tot, double *out) {
   const int nb = nsl_spectrum_len(s);
-  const unsigned nblk = (unsigned)std::min<long long>(std::max<long long>((s->x.g.D + 4095) / 4096, 1), 1024);
-  TmpBuf part, tot;
-  HIP_TRY(hipMalloc(&part.p, (size_t)nblk * nb * sizeof(double)));
-  HIP_TRY(hipMalloc(&tot.p, (size_t)nb * sizeof(double)));
-  hipLaunchKernelGGL(k_spectrum, dim3(nblk), dim3(NSL_NT), 4 * nb * sizeof(double), s->st, s->S, s->x.g, nb, (double *)part.p);
+  const unsigned nblk = spectrum_blocks(s);
+  hipLaunchKernelGGL(k_spectrum, dim3(nblk), dim3(NSL_NT), 4 * nb * sizeof(double), s->st, s->S, s->x.g, nb, part);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nb), dim3(NSL_NT), 0, s->st, (const double *)part.p, (int)nblk, nb, (double *)tot.p);
+  hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)nb), dim3(NSL_NT), 0, s->st, (const double *)part, (int)nblk, nb, tot);
   LAUNCH_CHECK();
-  HIP_TRY(hipMemcpyAsync(out, tot.p, nb * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIP_TRY(hipMemcpyAsync(out, tot, nb * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
   return NSL_OK;
+}
+
+static int spectrum_dev(nsl_solver *s, double *out) {
+  TmpBuf buf;
+  HIP_TRY(hipMalloc(&buf.p, spectrum_scratch_doubles(s) * sizeof(double)));
+  double *part = (double *)buf.p;
+  return spectrum_run(s, part, part + (size_t)spectrum_blocks(s) * nsl_spectrum_len(s), out);
 }
 
 static int spectrum_impl(nsl_solver *s, double *Ek) { return spectrum_dev(s, Ek); }

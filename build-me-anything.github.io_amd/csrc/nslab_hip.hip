@@ -359,6 +359,7 @@ struct nsl_solver {
   int N = 0, ic = 0;
   nsl_pool *pool = nullptr;   // a slot view: its pool and slot index
   int slot = -1;
+  int dev = 0;                // the GPU the solver's memory and stream live on
   double nu = 0, cfl = 0, dtFixed = 0, icp[4] = {0, 0, 0, 0};
   Xform x;
   hipStream_t st = 0;
@@ -683,6 +684,7 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
   if (o->ic < 0 || o->ic > NSL_IC_TUBESZ) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
   nsl_solver *s = new nsl_solver();
   s->N = N; s->nu = o->nu; s->cfl = o->cfl; s->dtFixed = o->dt;
+  if (hipGetDevice(&s->dev) != hipSuccess) { delete s; return fail(NSL_ERR_HIP, "hipGetDevice failed"); }
   hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete s; return fail(NSL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
   int r = xform_init(s->x, geo_truncated(N), s->st);
@@ -765,6 +767,7 @@ __global__ void k_pool_gather(const char *__restrict__ base, size_t sb, size_t o
 
 struct nsl_pool {
   int N = 0, C = 0, nblk = 0;
+  int dev = 0;
   Xform x;
   hipStream_t st = 0;
   SlotLayout L;
@@ -808,6 +811,7 @@ static nsl_solver *pool_view(nsl_pool *p, char *base, Batch bt, int slot) {
   v->view = true;
   v->pool = p;
   v->slot = slot;
+  v->dev = p->dev;
   v->N = p->N;
   v->x = p->x;
   v->st = p->st;
@@ -825,6 +829,7 @@ static int pool_create_impl(int N, int C, nsl_pool **out) {
   if (C < 1 || C > 65535) return fail(NSL_ERR_ARG, "slots must be in 1..65535");
   nsl_pool *p = new nsl_pool();
   p->N = N; p->C = C;
+  if (hipGetDevice(&p->dev) != hipSuccess) { delete p; return fail(NSL_ERR_HIP, "hipGetDevice failed"); }
   hipError_t e = hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete p; return fail(NSL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
   int r = xform_init(p->x, geo_truncated(N), p->st);
@@ -1110,6 +1115,49 @@ static int spectral_impl(int N, int forward, const double *f, const double *re, 
 
 #include "nsl_diag.hpp"
 
+// ------------------------------------------------------------------------------------------------ no-allocation scratch
+// Device scratch of the entry points that allocate nothing (nsl_spectrum_noalloc): one array per GPU, held until the
+// process ends. Host code never passes `scratch` itself (in host code the name is a host-side stand-in, not a device
+// address): each device's address comes from hipGetSymbolAddress, looked up once per device with that device
+// current, and only that address reaches kernels. Every call on one GPU shares the array, so calls that overlap on
+// one GPU (other threads, other solvers' streams) race; the entry points synchronise before returning.
+#define NSL_NOALLOC_SCRATCH_BYTES (4u << 20)
+static __device__ __align__(256) double scratch[NSL_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+// The scratch address on device `dev`, looked up on the first call for that device with it current (the caller's
+// current device is restored afterwards).
+static int noalloc_scratch(int dev, double **ptr) {
+  static std::mutex mu;
+  static void *addr[64] = {nullptr};
+  if (dev < 0 || dev >= 64) return fail(NSL_ERR_STATE, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess || p == nullptr)
+      return fail(NSL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[dev] = p;
+  }
+  *ptr = (double *)addr[dev];
+  return NSL_OK;
+}
+
+// nsl_spectrum with its partials in the device scratch: the same kernels and launch shape, so the same bits.
+static int spectrum_noalloc_impl(nsl_solver *s, double *Ek) {
+  const size_t need = spectrum_scratch_doubles(s) * sizeof(double);
+  if (need > NSL_NOALLOC_SCRATCH_BYTES)
+    return fail(NSL_ERR_ARG, "spectrum of N = " + std::to_string(s->N) + " needs " + std::to_string(need) +
+                                 " bytes of scratch; the allocation-free entry point holds " +
+                                 std::to_string(NSL_NOALLOC_SCRATCH_BYTES) + " (use nsl_spectrum)");
+  double *base = nullptr;
+  TRY(noalloc_scratch(s->dev, &base));
+  return spectrum_run(s, base, base + (size_t)spectrum_blocks(s) * nsl_spectrum_len(s), Ek);
+}
+
 // ------------------------------------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -1242,6 +1290,11 @@ int nsl_spectrum_len(nsl_solver *s) {
 int nsl_spectrum(nsl_solver *s, double *Ek) {
   if (!s || !Ek) return fail(NSL_ERR_ARG, "null argument");
   return spectrum_impl(s, Ek);
+}
+
+int nsl_spectrum_noalloc(nsl_solver *s, double *Ek) {
+  if (!s || !Ek) return fail(NSL_ERR_ARG, "null argument");
+  return spectrum_noalloc_impl(s, Ek);
 }
 
 int nsl_interp_max(nsl_solver *s, int32_t top, int32_t iters, double out[5]) {

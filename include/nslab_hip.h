@@ -107,6 +107,12 @@ int nsl_set_divmax(nsl_solver *s, double divmax);
 /* solver.spectrum() (nslab.js:414-420) and its length ceil(sqrt(3) kc) + 2. */
 int nsl_spectrum_len(nsl_solver *s);
 int nsl_spectrum(nsl_solver *s, double *Ek);
+/* nsl_spectrum without any allocation. No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels and launch
+ * shape as nsl_spectrum, so the same bits; its per-workgroup partials live in a fixed 4 MiB device array per GPU
+ * (held until the process ends), which covers every grid up to 768^3; larger grids return NSL_ERR_ARG. Calls on one
+ * GPU share that array: calls that overlap in time on one GPU (other threads, other solvers) race. */
+int nsl_spectrum_noalloc(nsl_solver *s, double *Ek);
 /* solver.interpMax(top, iters) (nslab.js:421-536): out = {omMaxI, x, y, z, omMaxNode}. Needs the vorticity field of
  * a preceding nsl_diagnose. */
 int nsl_interp_max(nsl_solver *s, int32_t top, int32_t iters, double out[5]);

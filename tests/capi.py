@@ -97,6 +97,8 @@ def lib():
         L.nsl_tune_get.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
         L.nsl_tune_set.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
         L.nsl_tune_candidates.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32]
+        L.nsl_spectrum_noalloc.argtypes = [vp, dp]
+        L.nsl_device_memory.argtypes = [i64p, i64p]
         L.nsl_slice.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
         L.nsl_image_diag.argtypes = [vp, dp]
         L.nsl_cutoff_pile_up.argtypes = [ctypes.c_int32, dp, ctypes.c_int32, dp]
