@@ -125,9 +125,11 @@ def measure_tubes(N, steps):
     done = s.run(steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    ser = s.series
     r = {"workload": "tube pair along z %d^3, Re 2000, CFL 0.5, FP64" % N, "steps": done, "steps_per_s": done / el,
-         "step_roofline_frac": b_alg_step(N) * done / el / (HBM_PEAK_GBS * 1e9), "setup_s": setup,
-         "device_GB": s.device_bytes() / 1e9, "t": s.clock["t"], "omMax": s.last["omMax"]}
+         "step_roofline_frac": b_alg_step(N) * done / el / (HBM_PEAK_GBS * 1e9), "setup_s": setup, "run_s": el,
+         "device_GB": s.device_bytes() / 1e9, "t": s.clock["t"], "omMax": s.last["omMax"], "E": s.last["E"],
+         "max_ebal": max(ser["ebal"]), "max_zbal": max(ser["zbal"])}
     s.close()
     return r
 
@@ -269,6 +271,27 @@ def fp64_vector(r):
             "kernels": ker}
 
 
+def copy_ceiling_GBs(mib=2048, reps=5):
+    """This GPU's practical HBM ceiling: a device-to-device copy of `mib` MiB (read + write bytes over the mean of
+    `reps` timed copies, torch events on its stream)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
+
+
 def traffic_from_profile(kernel, N):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if one exists."""
     p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -279,6 +302,22 @@ def traffic_from_profile(kernel, N):
         return d.get(str(N), {}).get(kernel, {}).get("bytes_per_launch")
     except (ValueError, OSError):
         return None
+
+
+def serving_bound(sv, copy_GBs, rate):
+    """How far batching can take the served mix: its RK4 steps priced at B_alg per step, run at this GPU's copy
+    ceiling and at the 8 TB/s peak, against the measured sequential time; and the service rate against the arrival
+    rate (above it the queue, and with it the latency, grows with the number of requests)."""
+    spg = {int(N): n for N, n in sv["steps_per_grid"].items()}
+    alg = sum(n * b_alg_step(N) for N, n in spg.items())
+    t_copy, t_peak = alg / (copy_GBs * 1e9), alg / (HBM_PEAK_GBS * 1e9)
+    mu = sv["requests"] / sv["wall_s"]
+    return {"steps_per_grid": spg, "b_alg_GB": alg / 1e9,
+            "bound_s_at_copy_ceiling": t_copy, "bound_vs_sequential": sv["sequential_estimate_s"] / t_copy,
+            "bound_vs_sequential_at_peak": sv["sequential_estimate_s"] / t_peak,
+            "served_b_alg_GBs": alg / sv["wall_s"] / 1e9,
+            "arrival_rate_per_s": rate, "service_rate_per_s": mu,
+            "queue": "stable" if mu > rate else "arrivals outpace service: latency grows with the request count"}
 
 
 def main():
@@ -336,6 +375,8 @@ def main():
                    "N": a.N, "nu": 1 / 1600, "dt": a.dt, "ic": "tgv"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel, "
+                                       "committed; not measured in this run)" if traffic else None,
                      "alg_bytes_per_launch": k["alg_bytes"], "avg_launch_us": k["avg_us"]},
         "step_roofline_frac": main_r["step_roofline_frac"],
         "zfwd_fused_with_update": main_r["zfwd_fused"],
@@ -350,6 +391,9 @@ def main():
     if hb:
         out["hbm_bytes_per_step_measured"] = hb
         out["hbm_bytes_per_step_vs_b_alg"] = hb / b_alg_step(a.N)
+        out["hbm_bytes_per_step_source"] = "profiles/pmc_traffic.json (rocprofv3, committed; not measured in this run)"
+    copy = copy_ceiling_GBs()
+    out["hbm_copy_GBs"] = copy
     if not a.no_cpu:
         out["cpu_sizes"] = cpu_sizes()
     if extra:
@@ -364,7 +408,8 @@ def main():
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
                                    for n, v in extra["kernels"].items()}}
     if not a.no_tubes:
-        out["tubes"] = [measure_tubes(512, 10), measure_tubes(1024, 3)]
+        # config 3: 500 RK4 steps at 512^3 (the full run), 3 at 1024^3
+        out["tubes"] = [measure_tubes(512, 500), measure_tubes(1024, 3)]
     if not a.no_ensemble:
         e = node_leg(["ensemble", "--steps", "300", "--seq-per-n", "4"], 600)
         out["ensemble"] = e if "error" in e else {
@@ -393,6 +438,8 @@ def main():
             "sequential_sample_s": sv.get("sequential_sample_s"),
             "sequential_sample_estimate_s": sv.get("sequential_sample_estimate_s"),
             "served_equals_sequential": sv.get("served_equals_sequential")}
+        if "error" not in sv:
+            out["serving"].update(serving_bound(sv, copy, 2.0))
         # the same loop with half the requests Reynolds sweeps (one grid of 64..192, three Re of the sqrt(2) series)
         sm = node_leg(["serve", "--requests", str(max(4, a.serve_requests // 2)), "--rate", "2", "--seed", "2",
                        "--sweeps", "0.5", "--seq-sample", "2"], 900)
@@ -405,6 +452,8 @@ def main():
             "p99_latency_s": sm["p99_latency_s"], "wall_s": sm["wall_s"],
             "throughput_vs_sequential": sm["throughput_vs_sequential"],
             "served_equals_sequential": sm.get("served_equals_sequential")}
+        if "error" not in sm:
+            out["serving_mixed"].update(serving_bound(sm, copy, 2.0))
     print(json.dumps(out))
 
 

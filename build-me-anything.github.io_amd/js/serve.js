@@ -345,8 +345,11 @@ if (require.main === module) {
     const Ns = [...new Set(LADDERS.flat().concat(o.sweepFraction > 0 ? SWEEP_GRIDS : []))].sort((a, b) => a - b);
     const costs = singleSolveCosts(Ns);
     const seqAll = sequentialEstimate(out.results, costs);
+    // the work served: RK4 steps per grid over every rung (the bench prices them with the HBM traffic model)
+    const stepsPerGrid = {};
+    for (const r of out.results) for (const l of r.levels) stepsPerGrid[l.N] = (stepsPerGrid[l.N] || 0) + l.steps;
     Object.assign(out, { single_solve_costs: costs, sequential_estimate_s: seqAll,
-      throughput_vs_sequential: seqAll / out.wall_s });
+      throughput_vs_sequential: seqAll / out.wall_s, steps_per_grid: stepsPerGrid });
     const seq = arg('seq-sample', 0);
     if (seq > 0) {
       const reqs = requestSpec(o), sq = serveSequential(reqs, out.results, seq);
