@@ -271,25 +271,13 @@ def fp64_vector(r):
             "kernels": ker}
 
 
-def copy_ceiling_GBs(mib=2048, reps=5):
-    """This GPU's practical HBM ceiling: a device-to-device copy of `mib` MiB (read + write bytes over the mean of
-    `reps` timed copies, torch events on its stream)."""
-    import torch
-    n = mib * (1 << 20) // 8
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    y.copy_(x)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(reps):
-        y.copy_(x)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
-    del x, y
-    torch.cuda.empty_cache()
-    return 2 * n * 8 / (ms * 1e-3) / 1e9
+def copy_ceiling_GBs(mib=2048, reps=10):
+    """This GPU's practical HBM ceiling: the library's plain copy kernel over `mib` MiB, bytes read + written per
+    second over `reps` copies (HIP events)."""
+    import capi
+    v = ctypes.c_double(0.0)
+    capi.check(capi.lib().nsl_copy_probe(mib << 20, reps, ctypes.byref(v)))
+    return v.value
 
 
 def traffic_from_profile(kernel, N):
@@ -420,6 +408,12 @@ def main():
             "pool_equals_single": e.get("pool_equals_single"), "graphed": e["graphed"],
             "b_alg_GBs": sum(n * b_alg_step(N) for N, n in ((32, 64), (48, 32), (64, 24), (96, 8))) * 300
                          / e["run_s"] / 1e9}
+        # the same ensemble with every pool's step in one captured graph (one launch per ensemble step)
+        e1 = node_leg(["ensemble", "--steps", "300", "--seq-per-n", "0", "--one-graph"], 600)
+        if "error" not in e and "error" not in e1:
+            out["ensemble"]["one_graph"] = {"run_s": e1["run_s"], "solve_steps_per_s": e1["solve_steps_per_s"],
+                                            "speedup_vs_sequential": e["sequential_s"] / e1["run_s"]
+                                            if e.get("sequential_s") else None}
         if not a.no_cpu and "error" not in e:
             ce = cpu_ensemble()
             out["ensemble"]["cpu_all_cores"] = ce

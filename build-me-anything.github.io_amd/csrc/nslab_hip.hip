@@ -382,6 +382,7 @@ struct nsl_solver {
   double *phys = nullptr;   // [15][N^3]
   double *dpart = nullptr;  // diagnose partials
   int diag_valid = 0;
+  hipEvent_t evfence = nullptr;   // a pool's launch object: orders a group launch after the pool's own stream
   // optional per-kernel timing with HIP events on the solver's stream (nsl_profile)
   int prof = 0;
   std::vector<hipEvent_t> evs;
@@ -655,6 +656,7 @@ static void solver_free(nsl_solver *s) {
   if (s->hrecs) (void)hipHostFree(s->hrecs);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (auto e : s->evs) (void)hipEventDestroy(e);
+  if (s->evfence) (void)hipEventDestroy(s->evfence);
   if (!s->view) {
     if (s->block) (void)hipFree(s->block);
     xform_free(s->x);
@@ -852,6 +854,10 @@ static int pool_create_impl(int N, int C, nsl_pool **out) {
     return fail(NSL_ERR_HIP, "staging allocation failed");
   }
   p->lead = pool_view(p, p->base, Batch{(unsigned)C, p->L.bytes}, -1);
+  if (hipEventCreateWithFlags(&p->lead->evfence, hipEventDisableTiming) != hipSuccess) {
+    pool_free(p);
+    return fail(NSL_ERR_HIP, "hipEventCreate failed");
+  }
   for (int i = 0; i < C; i++) p->slots.push_back(pool_view(p, p->base + (size_t)i * p->L.bytes, Batch(), i));
   p->used.assign(C, 0);
   p->gen.assign(C, 0);
@@ -1006,6 +1012,105 @@ static int pool_collect_impl(nsl_pool *p, nsl_steprec *recs, int recs_per_slot, 
   return NSL_OK;
 }
 
+// ------------------------------------------------------------------------------------------------ pool groups
+// Pools of different N stepped as one: one RK4 step of every pool of the group is captured as one hipGraph (a
+// fork/join capture: each pool's stream joins the origin stream's capture, enqueues its step, and rejoins), so an
+// ensemble step is one graph launch whatever the mix of grids. A group launch runs each pool's chunk set-up, `steps`
+// replays of the graph and each pool's record gather on the group's stream; each pool's stream then waits for it, so
+// nsl_pool_ready / nsl_pool_collect of every pool work as after nsl_pool_launch. Slot arithmetic is unchanged (same
+// kernels, same launch shapes), so results are bit-identical to the pools stepped alone.
+struct nsl_group {
+  std::vector<nsl_pool *> pools;
+  hipStream_t st = 0;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+static void group_free(nsl_group *g) {
+  if (!g) return;
+  if (g->st) (void)hipStreamSynchronize(g->st);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->done) (void)hipEventDestroy(g->done);
+  for (hipEvent_t e : g->join) (void)hipEventDestroy(e);
+  if (g->st) (void)hipStreamDestroy(g->st);
+  delete g;
+}
+
+static int group_create_impl(nsl_pool *const *pools, int n, nsl_group **out) {
+  if (!pools || !out || n < 1) return fail(NSL_ERR_ARG, "a group needs at least one pool");
+  *out = nullptr;
+  std::set<nsl_pool *> seen;
+  for (int i = 0; i < n; i++) {
+    if (!pools[i]) return fail(NSL_ERR_ARG, "null pool");
+    if (!seen.insert(pools[i]).second) return fail(NSL_ERR_ARG, "a pool appears twice in the group");
+    if (pools[i]->lastChunk > 0) return fail(NSL_ERR_STATE, "a pool of the group has a chunk in flight");
+  }
+  nsl_group *g = new nsl_group();
+  g->pools.assign(pools, pools + n);
+  auto bail = [&](int rc) { group_free(g); return rc; };
+  if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess) return bail(fail(NSL_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g->done, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(NSL_ERR_HIP, "hipEventCreate failed"));
+  g->join.resize(n, nullptr);
+  for (int i = 0; i < n; i++)
+    if (hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming) != hipSuccess) return bail(fail(NSL_ERR_HIP, "hipEventCreate failed"));
+  for (nsl_pool *p : g->pools)
+    if (hipStreamSynchronize(p->st) != hipSuccess) return bail(fail(NSL_ERR_HIP, "hipStreamSynchronize failed"));
+  if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return bail(fail(NSL_ERR_HIP, "hipStreamBeginCapture failed"));
+  int rc = hipEventRecord(g->fork, g->st) == hipSuccess ? NSL_OK : fail(NSL_ERR_HIP, "hipEventRecord failed");
+  for (int i = 0; i < n && rc == NSL_OK; i++) {
+    nsl_pool *p = g->pools[i];
+    if (hipStreamWaitEvent(p->st, g->fork, 0) != hipSuccess) { rc = fail(NSL_ERR_HIP, "hipStreamWaitEvent failed"); break; }
+    rc = step_enqueue(p->lead);
+    if (rc == NSL_OK && (hipEventRecord(g->join[i], p->st) != hipSuccess || hipStreamWaitEvent(g->st, g->join[i], 0) != hipSuccess))
+      rc = fail(NSL_ERR_HIP, "capture join failed");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(g->st, &graph);
+  if (rc == NSL_OK && (ec != hipSuccess || graph == nullptr)) rc = fail(NSL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  if (rc == NSL_OK && hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0) != hipSuccess) rc = fail(NSL_ERR_HIP, "hipGraphInstantiate failed");
+  g->graph = graph;
+  (void)hipGetLastError();
+  if (rc != NSL_OK) return bail(rc);
+  *out = g;
+  return NSL_OK;
+}
+
+static int group_launch_impl(nsl_group *g, int steps) {
+  if (!g) return fail(NSL_ERR_ARG, "null group");
+  if (steps < 1 || steps > NSL_RECCAP) return fail(NSL_ERR_ARG, "steps per launch must be in 1..64");
+  for (nsl_pool *p : g->pools)
+    if (p->lastChunk > 0) return fail(NSL_ERR_STATE, "a pool of the group has a chunk in flight: collect it first");
+  // the group's stream starts after whatever each pool's stream holds (admissions, state uploads)
+  for (nsl_pool *p : g->pools) {
+    HIP_TRY(hipEventRecord(p->lead->evfence, p->st));
+    HIP_TRY(hipStreamWaitEvent(g->st, p->lead->evfence, 0));
+  }
+  for (nsl_pool *p : g->pools) {
+    hipLaunchKernelGGL(k_chunk_begin, dim3(1, 1, p->C), dim3(1), 0, g->st, p->lead->ctl, p->L.bytes);
+    LAUNCH_CHECK();
+  }
+  for (int i = 0; i < steps; i++) HIP_TRY(hipGraphLaunch(g->gexec, g->st));
+  for (nsl_pool *p : g->pools) {
+    hipLaunchKernelGGL(k_pool_gather, dim3(p->C), dim3(256), 0, g->st, (const char *)p->base, p->L.bytes, p->L.ctl,
+                       p->L.recs, steps, p->dstage);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(p->hstage, p->dstage, (size_t)p->C * sizeof(PoolStage), hipMemcpyDeviceToHost, g->st));
+  }
+  HIP_TRY(hipEventRecord(g->done, g->st));
+  for (nsl_pool *p : g->pools) {
+    HIP_TRY(hipStreamWaitEvent(p->st, g->done, 0));
+    p->lastChunk = steps;
+    p->launch_gen = p->gen;
+  }
+  return NSL_OK;
+}
+
 // ------------------------------------------------------------------------------------------------ state I/O
 static int get_state_impl(nsl_solver *s, double *const planes[6]) {
   const Geo &g = s->x.g;
@@ -1114,6 +1219,46 @@ static int spectral_impl(int N, int forward, const double *f, const double *re, 
 }
 
 #include "nsl_diag.hpp"
+
+// ------------------------------------------------------------------------------------------------ bandwidth probe
+// Plain device copy, four 16-byte elements per thread per trip (loads issued before the stores): the practical HBM
+// ceiling the benchmark reports beside the 8 TB/s peak.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, long long n4) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    double2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = a[i + q * n4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) b[i + q * n4] = v[q];
+  }
+}
+
+static int copy_probe_impl(int64_t bytes, int reps, double *gbs) {
+  const long long n4 = (long long)(bytes / (4 * (int64_t)sizeof(double2)));
+  if (n4 < 1 || reps < 1) return fail(NSL_ERR_ARG, "copy probe needs at least 64 bytes and one repetition");
+  TmpBuf a, b;
+  HIP_TRY(hipMalloc(&a.p, (size_t)n4 * 4 * sizeof(double2)));
+  HIP_TRY(hipMalloc(&b.p, (size_t)n4 * 4 * sizeof(double2)));
+  HIP_TRY(hipMemset(a.p, 0, (size_t)n4 * 4 * sizeof(double2)));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const unsigned grid = (unsigned)std::min<long long>((n4 + 255) / 256, 256LL * 64);
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const double2 *)a.p, (double2 *)b.p, n4);   // warm-up
+  HIP_TRY(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const double2 *)a.p, (double2 *)b.p, n4);
+  HIP_TRY(hipEventRecord(e1, 0));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  LAUNCH_CHECK();
+  *gbs = 2.0 * (double)n4 * 4 * sizeof(double2) * reps / (ms * 1e-3) / 1e9;
+  return NSL_OK;
+}
 
 // ------------------------------------------------------------------------------------------------ no-allocation scratch
 // Device scratch of the entry points that allocate nothing (nsl_spectrum_noalloc): one array per GPU, held until the
@@ -1333,6 +1478,11 @@ int nsl_grid_size(const nsl_solver *s) {
   return s->N;
 }
 
+int nsl_copy_probe(int64_t bytes, int32_t reps, double *gbs) {
+  if (!gbs) return fail(NSL_ERR_ARG, "null argument");
+  return copy_probe_impl(bytes, reps, gbs);
+}
+
 int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes) {
   size_t f = 0, t = 0;
   HIP_TRY(hipMemGetInfo(&f, &t));
@@ -1393,6 +1543,10 @@ int nsl_xpass_plan(int32_t N, const nsl_tune *cfg, int32_t cus, int64_t cap_pair
 }
 
 void nsl_pool_destroy(nsl_pool *p) { pool_free(p); }
+
+int nsl_group_create(nsl_pool *const *pools, int32_t n, nsl_group **out) { return group_create_impl(pools, n, out); }
+int nsl_group_launch(nsl_group *g, int32_t steps) { return group_launch_impl(g, steps); }
+void nsl_group_destroy(nsl_group *g) { group_free(g); }
 
 int64_t nsl_pool_slot_bytes(int32_t N) {
   std::vector<int> f;

@@ -281,7 +281,24 @@ class Pool {
   destroy() { addon.poolDestroy(this.h); this.members.clear(); }
 }
 
+/**
+ * Pools of different N stepped as one: one RK4 step of every pool is one captured hipGraph, so an ensemble step of
+ * uneven grids is one graph launch. launch(k) steps every pool; then each pool's ready()/collect() as usual. Destroy
+ * the group before any of its pools.
+ */
+class PoolGroup {
+  constructor(pools) {
+    this.pools = pools.slice();
+    this.h = addon.groupCreate(this.pools.map(p => p.h));
+  }
+  launch(k) {
+    for (const p of this.pools) { for (const s of p.members.values()) s._sync(); p.k = k || 64; }
+    addon.groupLaunch(this.h, k || 64);
+  }
+  destroy() { addon.groupDestroy(this.h); }
+}
+
 function deviceMemory() { const a = addon.deviceMemory(); return { free: a[0], total: a[1] }; }
 
-module.exports = { VERSION, IC_INFO, createSolver, studySummary, Spectral, fftPlan, fft1, factorize, Pool, deviceMemory,
+module.exports = { VERSION, IC_INFO, createSolver, studySummary, Spectral, fftPlan, fft1, factorize, Pool, PoolGroup, deviceMemory,
   addonVersion: addon.version() };

@@ -86,13 +86,25 @@ function runEnsemble(opts = {}) {
     entry.running = list.length;
     pools.push(entry);
   }
+  // opts.group: every pool's step in one captured graph (one launch per ensemble step); else a graph per pool, each
+  // on its own stream
+  const group = opts.group ? new NS.PoolGroup(pools.map(p => p.pool)) : null;
   const tB = performance.now();
   let chunks = 0;
-  while (stepPools(pools, chunk) > 0) chunks++;
+  if (group) {
+    while (pools.some(p => p.running > 0)) {
+      group.launch(chunk);
+      for (const p of pools) p.running = p.pool.collect();
+      chunks++;
+    }
+  } else {
+    while (stepPools(pools, chunk) > 0) chunks++;
+  }
   const tC = performance.now();
+  if (group) group.destroy();
   const solves = spec.length, stepsDone = pools.reduce((a, p) => a + p.members.reduce((b, m) => b + m.solver.st.step, 0), 0);
   const res = {
-    solves, steps, chunks, admit_s: (tB - tA) / 1e3, run_s: (tC - tB) / 1e3,
+    solves, steps, chunks, one_graph: !!group, admit_s: (tB - tA) / 1e3, run_s: (tC - tB) / 1e3,
     solve_steps_per_s: stepsDone / ((tC - tB) / 1e3), ensemble_steps_per_s: steps / ((tC - tB) / 1e3),
     graphed: pools.every(p => p.pool.info().graphed === 1),
     pools: pools.map(p => ({ N: p.N, slots: p.pool.slots, bytes: p.pool.info().bytes })),
@@ -335,7 +347,8 @@ if (require.main === module) {
   const arg = (name, d) => { const i = argv.indexOf('--' + name); return i >= 0 ? +argv[i + 1] : d; };
   let out;
   if (mode === 'ensemble') {
-    out = runEnsemble({ steps: arg('steps', 300), seqPerN: arg('seq-per-n', 8), chunk: arg('chunk', 64) });
+    out = runEnsemble({ steps: arg('steps', 300), seqPerN: arg('seq-per-n', 8), chunk: arg('chunk', 64),
+      group: argv.includes('--one-graph') });
   } else if (mode === 'serve') {
     const o = { requests: arg('requests', 60), rate: arg('rate', 2), seed: arg('seed', 1), chunk: arg('chunk', 32),
       budgetGB: arg('budget-gb', 250), sweepFraction: arg('sweeps', 0) };

@@ -550,6 +550,73 @@ static napi_value js_pool_destroy(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* A group of pools (nsl_group_*). The JS side keeps the pools referenced for the group's life and destroys the
+ * group before any of its pools. */
+typedef struct {
+  nsl_group *g;
+} group_ref;
+
+static void finalize_group(napi_env env, void *data, void *hint) {
+  (void)env; (void)hint;
+  group_ref *r = (group_ref *)data;
+  if (r->g) nsl_group_destroy(r->g);
+  free(r);
+}
+
+/* groupCreate([pool handles]) -> group handle */
+static napi_value js_group_create(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  uint32_t n = 0;
+  if (napi_get_array_length(env, argv[0], &n) != napi_ok || n == 0 || n > 1024) {
+    napi_throw_type_error(env, NULL, "groupCreate([pool, ...])");
+    return NULL;
+  }
+  nsl_pool *pools[1024];
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e;
+    CALL(env, napi_get_element(env, argv[0], i, &e));
+    pools[i] = get_pool(env, e);
+    if (!pools[i]) { napi_throw_type_error(env, NULL, "not a pool handle"); return NULL; }
+  }
+  nsl_group *g = NULL;
+  int rc = nsl_group_create(pools, (int32_t)n, &g);
+  if (rc < 0) return throw_status(env, rc);
+  group_ref *r = (group_ref *)malloc(sizeof(group_ref));
+  if (!r) { nsl_group_destroy(g); napi_throw_error(env, NULL, "out of host memory"); return NULL; }
+  r->g = g;
+  napi_value ext;
+  CALL(env, napi_create_external(env, r, finalize_group, NULL, &ext));
+  return ext;
+}
+
+static group_ref *get_group(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || p == NULL) return NULL;
+  return (group_ref *)p;
+}
+
+/* groupLaunch(g, k) */
+static napi_value js_group_launch(napi_env env, napi_callback_info info) {
+  ARGS(2);
+  group_ref *r = get_group(env, argv[0]);
+  if (!r || !r->g) { napi_throw_type_error(env, NULL, "not a group handle"); return NULL; }
+  int64_t k = 64;
+  get_i64(env, argv[1], &k);
+  int rc = nsl_group_launch(r->g, (int32_t)k);
+  if (rc < 0) return throw_status(env, rc);
+  return NULL;
+}
+
+/* groupDestroy(g): frees the group's graph now (its pools stay) */
+static napi_value js_group_destroy(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  group_ref *r = get_group(env, argv[0]);
+  if (!r) { napi_throw_type_error(env, NULL, "not a group handle"); return NULL; }
+  if (r->g) nsl_group_destroy(r->g);
+  r->g = NULL;
+  return NULL;
+}
+
 /* poolInfo(p) -> [N, slots, used, graphed, bytes] */
 static napi_value js_pool_info(napi_env env, napi_callback_info info) {
   ARGS(1);
@@ -770,6 +837,7 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"deviceMemory", js_device_memory}, {"poolReady", js_pool_ready},
     {"tuneGet", js_tune_get}, {"tuneSet", js_tune_set}, {"tuneCandidates", js_tune_candidates},
     {"slice", js_slice}, {"imageDiag", js_image_diag}, {"cutoffPileUp", js_cutoff_pile_up},
+    {"groupCreate", js_group_create}, {"groupLaunch", js_group_launch}, {"groupDestroy", js_group_destroy},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

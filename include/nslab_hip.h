@@ -189,6 +189,10 @@ int nsl_profile_read(nsl_solver *s, double *ms, int64_t *count);
  * re-targeted after the launch were not part of it and may be set up while it runs. */
 typedef struct nsl_pool nsl_pool;
 int64_t nsl_pool_slot_bytes(int32_t N);
+/* HBM copy rate of the current GPU (no reference counterpart: the practical ceiling the benchmark reports beside the
+ * 8 TB/s peak): `reps` copies of a `bytes` buffer by a plain copy kernel, *gbs = bytes read + written per second / 1e9.
+ * Allocates and frees two buffers of `bytes`. */
+int nsl_copy_probe(int64_t bytes, int32_t reps, double *gbs);
 /* Free and total device memory of the current GPU (the admission budget of a request loop). */
 int nsl_device_memory(int64_t *free_bytes, int64_t *total_bytes);
 int nsl_pool_create(int32_t N, int32_t slots, nsl_pool **out);
@@ -201,6 +205,21 @@ int nsl_pool_launch(nsl_pool *p, int32_t steps);
 int nsl_pool_ready(nsl_pool *p);
 int nsl_pool_collect(nsl_pool *p, nsl_steprec *recs, int32_t recs_per_slot, int64_t *n_done, int32_t *n_running);
 int nsl_pool_release(nsl_pool *p, int32_t slot);
+
+/* Pool groups: pools of different N stepped as one. The reference steps one solve at a time (nslab-worker.js:42-49);
+ * an ensemble of uneven grids is here one pool per N, and a group makes one RK4 step of all of them one captured
+ * hipGraph (a fork/join capture of every pool's step), so an ensemble step is a single graph launch.
+ *   nsl_group_create(pools, n, &g)  captures the graph (no pool may have a chunk in flight); the pools stay usable
+ *                                   alone, and must outlive the group
+ *   nsl_group_launch(g, k)          k <= 64 steps of every slot with steps left in every pool of the group; returns
+ *                                   at once; then nsl_pool_ready / nsl_pool_collect on each pool as after
+ *                                   nsl_pool_launch (each pool's stream waits for the group's work)
+ *   nsl_group_destroy(g)
+ * Slot results are bit-identical to the same pools launched one by one. */
+typedef struct nsl_group nsl_group;
+int nsl_group_create(nsl_pool *const *pools, int32_t n, nsl_group **out);
+int nsl_group_launch(nsl_group *g, int32_t steps);
+void nsl_group_destroy(nsl_group *g);
 
 /* NS.fft1 / NS.Spectral (nslab.js:46-155) on the GPU: `lines` independent length-N lines (split re/im, line-major)
  * transformed in place, sign -1 forward, +1 inverse, unnormalised. */

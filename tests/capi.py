@@ -98,6 +98,7 @@ def lib():
         L.nsl_tune_set.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune)]
         L.nsl_tune_candidates.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32]
         L.nsl_spectrum_noalloc.argtypes = [vp, dp]
+        L.nsl_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int32, dp]
         L.nsl_device_memory.argtypes = [i64p, i64p]
         L.nsl_slice.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
         L.nsl_image_diag.argtypes = [vp, dp]

@@ -46,6 +46,26 @@ const sameSeries = (a, b) => ['t', 'E', 'Z', 'eps', 'omMax', 'uMax', 'dt', 'ebal
   const r = SV.runEnsemble({ mix: [[32, 6], [48, 3]], steps: 40, seqPerN: 2, chunk: 16 });
   check('ensemble: every solve stepped', r.allStepped && r.solves === 9);
   check('ensemble: pool slot equals the single solve', r.pool_equals_single);
+  // every pool's step in one captured graph: the same bits as a graph per pool
+  const g = SV.runEnsemble({ mix: [[32, 6], [48, 3], [64, 2]], steps: 40, seqPerN: 2, chunk: 16, group: true });
+  check('ensemble in one graph: every solve stepped', g.allStepped && g.solves === 11 && g.one_graph);
+  check('ensemble in one graph: pool slot equals the single solve', g.pool_equals_single);
+}
+
+// 2b. _S of a pool member follows the device: read, step through the pool, read again
+{
+  const pool = new NS.Pool(32, 2);
+  const a = pool.admit({ Re: 800, ic: 'random', cfl: 0.5, icParams: { seed: 9 } });
+  const before = Float64Array.from(a._S[0]);
+  pool.setTarget(a, 7);
+  while (true) { pool.launch(4); if (pool.collect() === 0) break; }
+  const after = a._S[0];
+  const ref = NS.createSolver({ N: 32, Re: 800, ic: 'random', cfl: 0.5, icParams: { seed: 9 } });
+  ref.run(7);
+  check('pool member _S after stepping is the stepped state', after.some((v, i) => v !== before[i]) &&
+    after.every((v, i) => v === ref._S[0][i]) && a.st.step === 7);
+  ref.destroy();
+  pool.destroy();
 }
 
 // 3. the request loop: a request served alone and in a mix gives the same levels, equal to sequential solves
