@@ -53,6 +53,46 @@ check('parity gate fails a drifted run', parityCheck(bad, fa, () => {}).verdict 
 const cmp = quiet(() => RL.runLong(args(path.join(base, 'c'), ['--compare', path.join(A, 'final.json')])));
 check('--compare against the archive: zero difference', Object.values(cmp.compare).every(v => v === 0) && cmp.steps === fa.steps + 1,
   JSON.stringify(cmp));
+
+// the reference's formats (tests/golden/formats.json: key schema of archived final.json files of both reference
+// writers, the checkpoint.json keys they write, the fields research/nslab/analyse.js reads)
+const F = JSON.parse(fs.readFileSync(path.join(__dirname, '..', 'golden', 'formats.json'), 'utf8'));
+const has = (obj, ks) => ks.filter(k => !(k in obj));
+const jsType = v => (Array.isArray(v) ? 'list' : v === null ? 'null' : typeof v === 'object' ? 'dict' : typeof v === 'string' ? 'str' :
+  typeof v === 'boolean' ? 'bool' : 'number');
+for (const [label, file] of [['final.json', 'final.json'], ['partial.json', 'partial.json']]) {
+  const j = JSON.parse(fs.readFileSync(path.join(A, file), 'utf8'));
+  for (const w of ['final_cpu', 'final_gpu']) {
+    const S = F[w], miss = [].concat(has(j, S.top), has(j.case, S.case).map(k => 'case.' + k),
+      has(j.snapshots[j.snapshots.length - 1], S.snapshot).map(k => 'snapshot.' + k),
+      has(j.health, S.health).map(k => 'health.' + k), has(j.series, S.series).map(k => 'series.' + k));
+    const badType = S.top.filter(k => k in j && !(jsType(j[k]) === S.types[k] || (jsType(j[k]) === 'number' && ['int', 'float'].includes(S.types[k]))));
+    check(`${label} carries every key of the reference writer's ${w.replace('final_', '')} format`, miss.length === 0 && badType.length === 0,
+      miss.concat(badType.map(k => 'type ' + k)).join(' ') || null);
+  }
+  const R = F.analyse_reads, missA = [].concat(has(j, R.top), has(j.case, R.case).map(k => 'case.' + k),
+    has(j.health, R.health).map(k => 'health.' + k), has(j.series, R.series).map(k => 'series.' + k), has(j.build, R.build).map(k => 'build.' + k));
+  check(`${label} carries every field analyse.js reads`, missA.length === 0, missA.join(' ') || null);
+}
+const ck = JSON.parse(fs.readFileSync(path.join(A, 'checkpoint.json'), 'utf8'));
+check('checkpoint.json carries the keys of both reference writers', has(ck, F.checkpoint_cpu).length === 0 && has(ck, F.checkpoint_gpu).length === 0,
+  has(ck, F.checkpoint_gpu).join(' ') || null);
+const logRe = /^\[\d{4}-\d\d-\d\d \d\d:\d\d:\d\d\] NSLab \S+ \(.*\) — tgv N=32³ Re=1600 tEnd=1\.2 cfl=0\.4 snap=0\.5 ckpt=0\.4  /;
+check('run.log opens with the reference\'s line shape', logRe.test(fs.readFileSync(path.join(A, 'run.log'), 'utf8').split('\n')[0]) &&
+  /^\[\d{4}-\d\d-\d\d \d\d:\d\d:\d\d\] NSLab /.test(F.log_cpu));
+
+// --compare against a series the reference itself wrote (TGV 32^3, Re 1600, CFL 0.4 to t = 2, tests/golden/), wrapped in
+// the final.json schema: every series within 1e-13 of the reference's scale
+{
+  const g = JSON.parse(fs.readFileSync(path.join(__dirname, '..', 'golden', 'tgv32_cfl04_t2.json'), 'utf8'));
+  const wrapped = path.join(base, 'ref_final.json');
+  fs.writeFileSync(wrapped, JSON.stringify({ case: { ic: 'tgv', N: 32, Re: 1600, nu: 1 / 1600, cfl: 0.4, tEnd: 2 }, t: g.st.t,
+    steps: g.st.step, series: g.series, final: true }));
+  const c = quiet(() => RL.runLong(RL.parseArgs(['--N', '32', '--Re', '1600', '--tEnd', '2', '--cfl', '0.4', '--out', path.join(base, 'r'),
+    '--compare', wrapped])));
+  check('--compare against the reference-written TGV 32^3 series: within 1e-13', c.steps === g.series.t.length &&
+    Object.values(c.compare).every(v => v < 1e-13), JSON.stringify(c));
+}
 fs.rmdirSync(base, { recursive: true });
 console.log(fails ? `${fails} FAILED` : 'ALL PASS');
 process.exit(fails ? 1 : 0);
