@@ -19,6 +19,11 @@
 static constexpr int fast_T(int N) {
   return N <= 64 ? ((N / 3 + 1) + 7) & ~7 : N == 128 ? 16 : N <= 256 ? NSL_T_SMALL : 8;
 }
+// The other tile width the autotuner may pick (pt = 1), or 0: it must divide the row pitch hp (whole tiles) and fit
+// LDS (N x T complex). (16 lines at 512^3 would spill the z-inverse: 476 bytes of scratch per lane.)
+static constexpr int alt_T(int N) {
+  return N == 32 || N == 48 || N == 64 ? 8 : N == 96 ? 40 : N == 128 ? 24 : N == 192 ? 24 : 0;
+}
 // Compute units of the current device (cached per device).
 static int device_cus() {
   static std::mutex mu;
@@ -45,8 +50,8 @@ static bool fast_setup(Xform &x) {
     NSL_FAST_SIZES(NSL_CASE)
 #undef NSL_CASE
     x.fast = true;
-    x.fT = fast_T(x.g.N);
     x.tune = tune_for(x.g.N);
+    x.fT = x.tune.pt ? alt_T(x.g.N) : fast_T(x.g.N);
     return true;
     default:
       x.fast = false;
@@ -56,17 +61,15 @@ static bool fast_setup(Xform &x) {
 
 static int fast_ntile(const Xform &x) { return (x.g.hx + x.fT - 1) / x.fT; }
 
-template <int N, int MODE>
+template <int N, int T, int MODE>
 static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int nout, int gc, const Ctl *ctl, const Batch &bt) {
-  constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_zinv<N, T, MODE>, lds);
   hipLaunchKernelGGL((fast::kf_zinv<N, T, MODE>), dim3((unsigned)(nout * fast_ntile(x) * x.g.by), 1, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, U, X1, x.g, x.twN, nout, gc, ctl, bt.sb);
 }
 
-template <int N>
-static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
-  constexpr int T = fast_T(N);
+template <int N, int T>
+static int fast_zinv_nt(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
   const int lds = N * T * (int)sizeof(double2);
   if (mode == 1 && nout == 6 && N <= NSL_ZINV_UW_MAXN) {   // the rhs: u and omega from five transforms per tile
     // (4-line tiles at 1024^3, two workgroups per CU: 12.1 against 8.8 ms)
@@ -75,25 +78,23 @@ static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, 
     LAUNCH_CHECK();
     return NSL_OK;
   }
-  if (mode == 0) kf_zinv_launch<N, 0>(x, U, X1, nout, gc, ctl, bt);
-  else if (mode == 1) kf_zinv_launch<N, 1>(x, U, X1, nout, gc, ctl, bt);
-  else kf_zinv_launch<N, 2>(x, U, X1, nout, gc, ctl, bt);
+  if (mode == 0) kf_zinv_launch<N, T, 0>(x, U, X1, nout, gc, ctl, bt);
+  else if (mode == 1) kf_zinv_launch<N, T, 1>(x, U, X1, nout, gc, ctl, bt);
+  else kf_zinv_launch<N, T, 2>(x, U, X1, nout, gc, ctl, bt);
   LAUNCH_CHECK();
   return NSL_OK;
 }
-template <int N>
-static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
-  constexpr int T = fast_T(N);
+template <int N, int T>
+static int fast_yinv_nt(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_yinv<N, T>, lds);
   hipLaunchKernelGGL((fast::kf_yinv<N, T>), dim3((unsigned)(fast_ntile(x) * N), (unsigned)ncomp, bt.nz), dim3(fast::PassNT<N>::v), lds, x.st, X1, X2, x.g, x.twN, ctl, bt.sb);
   LAUNCH_CHECK();
   return NSL_OK;
 }
-template <int N>
-static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
+template <int N, int T>
+static int fast_yfwd_nt(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
                        const double *pmax, int nblk) {
-  constexpr int T = fast_T(N);
   const int lds = N * T * (int)sizeof(double2);
   allow_lds(fast::kf_yfwd<N, T>, lds);
   const unsigned grid = (unsigned)(fast_ntile(x) * N);
@@ -103,9 +104,8 @@ static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp
   LAUNCH_CHECK();
   return NSL_OK;
 }
-template <int N>
-static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
-  constexpr int T = fast_T(N);
+template <int N, int T>
+static int fast_zfwd_nt(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
   const int lds = N * T * (int)sizeof(double2);
   const double inv = 1.0 / ((double)N * N * N);
   allow_lds(fast::kf_zfwd<N, T>, lds);
@@ -113,6 +113,29 @@ static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncom
   LAUNCH_CHECK();
   return NSL_OK;
 }
+// The pencil passes at the solver's tile width x.fT (fast_T(N), or alt_T(N) when the autotuner picked it).
+template <int N>
+static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
+  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_zinv_nt<N, alt_T(N)>(x, U, X1, mode, nout, gc, ctl, bt);
+  return fast_zinv_nt<N, fast_T(N)>(x, U, X1, mode, nout, gc, ctl, bt);
+}
+template <int N>
+static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
+  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_yinv_nt<N, alt_T(N)>(x, X1, X2, ncomp, ctl, bt);
+  return fast_yinv_nt<N, fast_T(N)>(x, X1, X2, ncomp, ctl, bt);
+}
+template <int N>
+static int fast_yfwd_n(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
+                       const double *pmax, int nblk) {
+  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_yfwd_nt<N, alt_T(N)>(x, X3, X4, ncomp, ctl, bt, pmax, nblk);
+  return fast_yfwd_nt<N, fast_T(N)>(x, X3, X4, ncomp, ctl, bt, pmax, nblk);
+}
+template <int N>
+static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
+  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_zfwd_nt<N, alt_T(N)>(x, X4, OUT, ncomp, ctl, bt);
+  return fast_zfwd_nt<N, fast_T(N)>(x, X4, OUT, ncomp, ctl, bt);
+}
+
 // x-pass forms compiled for each N. Row pairs: N <= 384, LP row pairs per workgroup with 6*LP*N complex of LDS
 // (at most 160 KB). Rows: N >= 384. One wave per row group (plain or staged): N = 256, 512.
 enum { NSL_XP_PAIR = 1, NSL_XP_ROWS = 2, NSL_XP_WAVE = 3, NSL_XP_STAGED = 4 };
@@ -124,6 +147,14 @@ static constexpr bool xp_has(int N, int form) {
        : form == NSL_XP_STAGED ? (N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 ||
                                   N == 384 || N == 512 || N == 1024)
        : false;
+}
+// sizes with a variant-1 schedule of the staged x pass (XnlSched) and of the fused update (ZuSchedV)
+static constexpr bool xs_has(int N) {
+  return N == 32 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 || N == 1024;
+}
+static constexpr bool zs_has(int N) {
+  return N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 ||
+         N == 1024;
 }
 static constexpr bool zu_has(int N) {
   return N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 ||
@@ -137,10 +168,11 @@ static long long xnl_workgroups(const TuneCfg &c, int cus) {
   switch (c.xpass) {
     case NSL_XP_STAGED:
       if constexpr (xp_has(N, NSL_XP_STAGED)) {
-        using C = fast::XnlStaged<N>;
-        long long grid = (long long)N * N / C::RPW;
+        const int RPW = c.xs ? fast::XnlStaged<N, 1>::RPW : fast::XnlStaged<N, 0>::RPW;
+        const int WPR = c.xs ? fast::XnlStaged<N, 1>::WPR : fast::XnlStaged<N, 0>::WPR;
+        long long grid = (long long)N * N / RPW;
         // persistent: xwaves waves per CU walk the row groups
-        if (c.xwaves > 0) grid = std::min<long long>(grid, std::max(1, cus * c.xwaves / C::WPR));
+        if (c.xwaves > 0) grid = std::min<long long>(grid, std::max(1, cus * c.xwaves / WPR));
         return grid;
       }
       return 0;
@@ -191,8 +223,11 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
   switch (c.xpass) {
     case NSL_XP_STAGED:
       if constexpr (xp_has(N, NSL_XP_STAGED)) {
-        using C = fast::XnlStaged<N>;
-        hipLaunchKernelGGL((fast::kf_xnl_dma<N>), dim3((unsigned)grid, 1, bt.nz), dim3(C::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        if (c.xs == 0) {
+          hipLaunchKernelGGL((fast::kf_xnl_dma<N, 0>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 0>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        } else if constexpr (xs_has(N)) {
+          hipLaunchKernelGGL((fast::kf_xnl_dma<N, 1>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 1>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        }
       }
       break;
     case NSL_XP_WAVE:
@@ -275,15 +310,23 @@ static int fast_zu_blocks(const Geo &g) {
   const int T = fast::zu_lines(g.N);   // fast::ZuCfg<N>::T
   return ((g.hx + T - 1) / T) * g.by;
 }
-template <int N, int STAGE>
-static void fast_zu_launch(const Xform &x, const double2 *X4, const double2 *U, double2 *S, double2 *ACC, double2 *TS,
-                           double *partials, Ctl *ctl, const Batch &bt) {
+template <int N, int STAGE, int V>
+static void fast_zu_launch_v(const Xform &x, const double2 *X4, const double2 *U, double2 *S, double2 *ACC, double2 *TS,
+                             double *partials, Ctl *ctl, const Batch &bt) {
   using C = fast::ZuCfg<N>;
   const int lds = N * C::T * (int)sizeof(double2);
   const double inv = 1.0 / ((double)N * N * N);
-  allow_lds(fast::kf_zfwd_upd<N, STAGE>, lds);
-  hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE>), dim3((unsigned)fast_zu_blocks(x.g), 1, bt.nz), dim3(C::NT), lds, x.st,
+  allow_lds(fast::kf_zfwd_upd<N, STAGE, V>, lds);
+  hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE, V>), dim3((unsigned)fast_zu_blocks(x.g), 1, bt.nz), dim3(C::NT), lds, x.st,
                      X4, U, S, ACC, TS, x.g, x.twN, inv, partials, ctl, bt.sb);
+}
+// the fused update with the tuned schedule variant (zs)
+template <int N, int STAGE>
+static void fast_zu_launch(const Xform &x, const double2 *X4, const double2 *U, double2 *S, double2 *ACC, double2 *TS,
+                           double *partials, Ctl *ctl, const Batch &bt) {
+  if constexpr (zs_has(N))
+    if (x.tune.zs) return fast_zu_launch_v<N, STAGE, 1>(x, X4, U, S, ACC, TS, partials, ctl, bt);
+  fast_zu_launch_v<N, STAGE, 0>(x, X4, U, S, ACC, TS, partials, ctl, bt);
 }
 template <int N>
 static int fast_zu_n(const Xform &x, int stage, const double2 *X4, const double2 *U, double2 *S, double2 *ACC,

@@ -272,6 +272,19 @@ template <> struct ZuSched<256> { using L = RList<8, 4, 8>; };
 template <> struct ZuSched<384> { using L = RList<16, 3, 8>; };
 template <> struct ZuSched<512> { using L = RList<8, 8, 8>; };
 template <> struct ZuSched<1024> { using L = RList<16, 8, 8>; };
+// Alternative schedules the autotuner may pick instead (variant 1; same last radix 8, so the same thread-to-mode
+// ownership): more, smaller stages at the front.
+template <int N, int V> struct ZuSchedV { using L = typename ZuSched<N>::L; };
+template <> struct ZuSchedV<32, 1> { using L = RList<2, 2, 8>; };
+template <> struct ZuSchedV<48, 1> { using L = RList<6, 8>; };
+template <> struct ZuSchedV<64, 1> { using L = RList<4, 2, 8>; };
+template <> struct ZuSchedV<96, 1> { using L = RList<2, 6, 8>; };
+template <> struct ZuSchedV<128, 1> { using L = RList<4, 4, 8>; };
+template <> struct ZuSchedV<192, 1> { using L = RList<4, 6, 8>; };
+template <> struct ZuSchedV<256, 1> { using L = RList<4, 8, 8>; };
+template <> struct ZuSchedV<384, 1> { using L = RList<8, 6, 8>; };
+template <> struct ZuSchedV<512, 1> { using L = RList<16, 4, 8>; };
+template <> struct ZuSchedV<1024, 1> { using L = RList<8, 16, 8>; };
 #ifndef NSL_ZU_T
 #define NSL_ZU_T 8
 #endif
@@ -306,7 +319,7 @@ template <int N> struct ZuSlots {
     return 0;
   }
 };
-template <int N, int STAGE>
+template <int N, int STAGE, int V = 0>
 __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(const double2 *__restrict__ X4, const double2 *U,
                                                             double2 *S, double2 *__restrict__ ACC, double2 *TS,
                                                             const Geo, const double2 *__restrict__ tw, double inv,
@@ -330,7 +343,7 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
       return kx0 + t < g.hx ? Ib[((size_t)c * N + n) * plane + t] : make_double2(0.0, 0.0);
     };
     auto keep = [&](int, int, double2 v, int, int j) { Nl[c][j] = make_double2(v.x * inv, v.y * inv); };
-    fft_sched<N, T, NT, -1, false, Interleaved<N, T>, typename ZuSched<N>::L>(lds, tw, load, keep);
+    fft_sched<N, T, NT, -1, false, Interleaved<N, T>, typename ZuSchedV<N, V>::L>(lds, tw, load, keep);
   }
   const double nu = ctl->nu;
   const double dt = STAGE == 1 ? choose_dt(ctl, sqrt(__longlong_as_double((long long)ctl->u2))) : ctl->dt;
@@ -748,8 +761,20 @@ __host__ __device__ constexpr int gcd_int(int a, int b) { return b == 0 ? a : gc
 // butterflies per line, and the smallest workgroup of whole waves that holds whole rows, lcm(NB, 64) threads: one
 // wave for RPW = 64/NB rows up to 512^3, two waves per row at 1024^3, three waves for 8 or 16 rows at 48, 96, 192
 // and 384.
-template <int N> struct XnlStaged {
-  static constexpr int R = FirstR<typename XnlWave<N>::L>::R, NB = N / R;
+// Alternative schedules of the staged x pass the autotuner may pick (variant 1; same first and last radix, which
+// sets the workgroup's shape: NT = lcm(N/R, 64) threads).
+template <int N, int V> struct XnlSched { using L = typename XnlWave<N>::L; };
+template <> struct XnlSched<32, 1> { using L = RList<2, 8, 2>; };
+template <> struct XnlSched<64, 1> { using L = RList<4, 4, 4>; };
+template <> struct XnlSched<96, 1> { using L = RList<4, 3, 2, 4>; };
+template <> struct XnlSched<128, 1> { using L = RList<4, 8, 4>; };
+template <> struct XnlSched<192, 1> { using L = RList<4, 3, 4, 4>; };
+template <> struct XnlSched<256, 1> { using L = RList<4, 16, 4>; };
+template <> struct XnlSched<384, 1> { using L = RList<8, 3, 2, 8>; };
+template <> struct XnlSched<512, 1> { using L = RList<8, 4, 2, 8>; };
+template <> struct XnlSched<1024, 1> { using L = RList<8, 16, 8>; };
+template <int N, int V = 0> struct XnlStaged {
+  static constexpr int R = FirstR<typename XnlSched<N, V>::L>::R, NB = N / R;
   static constexpr int NT = NB / gcd_int(NB, 64) * 64, RPW = NT / NB, WPR = NT / 64;
   // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
   // (three waves per SIMD, 168 registers, spills 95 at 192^3 and 111 at 384^3)
@@ -759,15 +784,15 @@ template <int N> struct XnlStaged {
   // (two waves per SIMD or one), at 192^3 it spills 14 registers
   static constexpr bool TWC = NT == 64;
 };
-template <int N>
-__global__ __launch_bounds__(XnlStaged<N>::NT, XnlStaged<N>::MINW) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
+template <int N, int V = 0>
+__global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
                                                                const double2 *__restrict__ tw,
                                                                double *__restrict__ pmax, Ctl *__restrict__ ctl,
                                                                size_t sb) {
-  using C = XnlStaged<N>;
+  using C = XnlStaged<N, V>;
   constexpr int R = C::R, NB = C::NB, NT = C::NT, RPW = C::RPW, WPR = C::WPR;
   static_assert(NB * RPW == NT, "first and last stage: one butterfly per thread");
-  using SCH = typename XnlWave<N>::L;
+  using SCH = typename XnlSched<N, V>::L;
   // the forward transforms start from the inverse's last-stage registers: slot j of the radix-R last stage holds
   // element bb + j*N/R, which is what a radix-R first stage reads, so the schedule starts and ends in radix R
   static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in the same radix");

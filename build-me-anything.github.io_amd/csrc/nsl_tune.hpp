@@ -2,7 +2,7 @@
 //
 // A solve's kernels depend on N only, so a solve runs the same arithmetic alone and in any pool. The forms of each
 // N come, in order of precedence, from an override set through nsl_tune_set, from the file named by NSLAB_TUNE
-// (read once; lines "N xpass lp xwaves zfused", '#' starts a comment), from the table the tuner (js/tune.js) wrote
+// (read once; lines "N xpass lp xwaves zfused [pt xs zs]", '#' starts a comment), from the table the tuner (js/tune.js) wrote
 // (nsl_tune_table.inc), and from the defaults below for sizes the table does not list.
 #pragma once
 #include <cstdlib>
@@ -11,7 +11,7 @@
 #include <sstream>
 
 struct TuneRow {
-  int N, xpass, lp, xwaves, zfused;
+  int N, xpass, lp, xwaves, zfused, pt, xs, zs;
 };
 static const TuneRow kTuneTable[] = {
 #include "nsl_tune_table.inc"
@@ -24,6 +24,9 @@ static bool tune_valid(int N, const TuneCfg &c) {
   if (!xp_has(N, c.xpass)) return false;
   if (c.xpass == NSL_XP_PAIR && !((c.lp == 1 || c.lp == 2 || c.lp == 4 || c.lp == 8) && xp_pair_lp_ok(N, c.lp))) return false;
   if (c.xpass == NSL_XP_STAGED && (c.xwaves < 0 || c.xwaves > 256)) return false;
+  if ((c.pt != 0 && c.pt != 1) || (c.pt && alt_T(N) == 0)) return false;
+  if ((c.xs != 0 && c.xs != 1) || (c.xs && !(c.xpass == NSL_XP_STAGED && xs_has(N)))) return false;
+  if ((c.zs != 0 && c.zs != 1) || (c.zs && !(c.zfused && zs_has(N)))) return false;
   return true;
 }
 
@@ -61,7 +64,9 @@ static void tune_read_file_locked() {
     std::istringstream is(line);
     int N;
     TuneCfg c;
-    if (is >> N >> c.xpass >> c.lp >> c.xwaves >> c.zfused && tune_valid(N, c)) g_tune_file[N] = c;
+    if (!(is >> N >> c.xpass >> c.lp >> c.xwaves >> c.zfused)) continue;
+    if (!(is >> c.pt >> c.xs >> c.zs)) c.pt = c.xs = c.zs = 0;   // the five-field form: default tile and schedules
+    if (tune_valid(N, c)) g_tune_file[N] = c;
   }
 }
 
@@ -74,28 +79,37 @@ static TuneCfg tune_for(int N) {
   if (f != g_tune_file.end()) return f->second;
   for (const TuneRow &r : kTuneTable) {
     TuneCfg c;
-    c.xpass = r.xpass; c.lp = r.lp; c.xwaves = r.xwaves; c.zfused = r.zfused;
+    c.xpass = r.xpass; c.lp = r.lp; c.xwaves = r.xwaves; c.zfused = r.zfused; c.pt = r.pt; c.xs = r.xs; c.zs = r.zs;
     if (r.N == N && tune_valid(N, c)) return c;
   }
   return tune_default(N);
 }
 
-// Every compiled form of N, the tuner's search space.
+// Every compiled form of N, the tuner's search space: the x-pass forms (with their tile or persistence), fused or
+// two-kernel z-forward, and, crossed with those, the pencil tile width and the schedule variants where N has them.
+// The default tile and schedules come first, in the order of the x-pass forms.
 static std::vector<TuneCfg> tune_candidates(int N) {
-  std::vector<TuneCfg> out;
-  const int zs[2] = {0, 1};
-  for (int z : zs) {
+  std::vector<TuneCfg> base;
+  for (int z : {0, 1}) {
     if (z && !zu_has(N)) continue;
     for (int lp : {1, 2, 4, 8}) {
       TuneCfg c; c.xpass = NSL_XP_PAIR; c.lp = lp; c.zfused = z;
-      if (tune_valid(N, c)) out.push_back(c);
+      if (tune_valid(N, c)) base.push_back(c);
     }
-    { TuneCfg c; c.xpass = NSL_XP_ROWS; c.zfused = z; if (tune_valid(N, c)) out.push_back(c); }
-    { TuneCfg c; c.xpass = NSL_XP_WAVE; c.zfused = z; if (tune_valid(N, c)) out.push_back(c); }
+    { TuneCfg c; c.xpass = NSL_XP_ROWS; c.zfused = z; if (tune_valid(N, c)) base.push_back(c); }
+    { TuneCfg c; c.xpass = NSL_XP_WAVE; c.zfused = z; if (tune_valid(N, c)) base.push_back(c); }
     for (int w : {0, 8, 16, 32, 64}) {
       TuneCfg c; c.xpass = NSL_XP_STAGED; c.xwaves = w; c.zfused = z;
-      if (tune_valid(N, c)) out.push_back(c);
+      if (tune_valid(N, c)) base.push_back(c);
     }
   }
+  std::vector<TuneCfg> out;
+  for (int pt : {0, 1})
+    for (int xs : {0, 1})
+      for (int zs : {0, 1})
+        for (TuneCfg c : base) {
+          c.pt = pt; c.xs = xs; c.zs = zs;
+          if (tune_valid(N, c)) out.push_back(c);
+        }
   return out;
 }

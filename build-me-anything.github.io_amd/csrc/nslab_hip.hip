@@ -139,6 +139,8 @@ static void allow_lds(K kernel, int bytes) {
 // the z-forward runs fused with the update.
 struct TuneCfg {
   int xpass = 0, lp = 0, xwaves = 0, zfused = 0;
+  int pt = 0, xs = 0, zs = 0;   // pencil tile width (0 default, 1 alternative); schedule variants of the staged
+                                // x pass and of the fused update
 };
 
 struct Xform {
@@ -1503,7 +1505,7 @@ static bool fast_size(int N) {
       return false;
   }
 }
-static nsl_tune to_abi(const TuneCfg &c) { return nsl_tune{c.xpass, c.lp, c.xwaves, c.zfused}; }
+static nsl_tune to_abi(const TuneCfg &c) { return nsl_tune{c.xpass, c.lp, c.xwaves, c.zfused, c.pt, c.xs, c.zs}; }
 
 int nsl_tune_get(int32_t N, nsl_tune *out) {
   if (!fast_size(N)) return fail(NSL_ERR_ARG, "no size-specialised kernels for N = " + std::to_string(N));
@@ -1520,6 +1522,7 @@ int nsl_tune_set(int32_t N, const nsl_tune *cfg) {
   }
   TuneCfg c;
   c.xpass = cfg->xpass; c.lp = cfg->lp; c.xwaves = cfg->xwaves; c.zfused = cfg->zfused;
+  c.pt = cfg->ptile; c.xs = cfg->xsched; c.zs = cfg->zsched;
   if (!tune_valid(N, c)) return fail(NSL_ERR_ARG, "kernel forms not compiled for N = " + std::to_string(N));
   g_tune_over[N] = c;
   return NSL_OK;
@@ -1536,6 +1539,7 @@ int nsl_xpass_plan(int32_t N, const nsl_tune *cfg, int32_t cus, int64_t cap_pair
   if (!cfg || !workgroups) return fail(NSL_ERR_ARG, "null argument");
   TuneCfg c;
   c.xpass = cfg->xpass; c.lp = cfg->lp; c.xwaves = cfg->xwaves; c.zfused = cfg->zfused;
+  c.pt = cfg->ptile; c.xs = cfg->xsched; c.zs = cfg->zsched;
   long long g = 0;
   TRY(xnl_plan_n(N, c, cus > 0 ? cus : device_cus(), cap_pairs, &g));
   *workgroups = g;

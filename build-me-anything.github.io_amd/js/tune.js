@@ -1,15 +1,19 @@
 'use strict';
 /*
- * Autotuner of the kernel forms per grid size. For each N it times every compiled form (x-pass form and its tile,
- * persistent waves of the staged x pass, z-forward fused with the update or not; nsl_tune_candidates) on this
- * device, checks that each gives the same step as the current form, and writes the table the library compiles in
- * (csrc/nsl_tune_table.inc) plus a JSON report. A form replaces the current one only when it is faster by more
- * than the noise margin, so re-running on the same device keeps the table stable. Up to --pool-max (256) each form
- * is also timed in a pool of slots (the batched path: one launch per pass covers every slot), and a form's score is
- * the geometric mean of its single-solve and pool speeds relative to the current form.
+ * Autotuner of the kernel forms per grid size. For each N it times the compiled forms (nsl_tune_candidates) on this
+ * device in two searches and writes the table the library compiles in (csrc/nsl_tune_table.inc) plus a JSON report:
+ *   1. the x-pass form and its tile (row pairs per workgroup, persistent waves of the staged x pass) and whether
+ *      the z-forward runs fused with the update, at the default pencil tile and radix schedules;
+ *   2. from that pick, coordinate descent over the pencil passes' tile width and the radix schedules of the staged
+ *      x pass and of the fused update: each one changed alone, the best change kept, twice.
+ * Every form is checked to give the same step as the current form. A form replaces the current one only when it is
+ * faster by more than the noise margin, so re-running on the same device keeps the table stable. Up to --pool-max
+ * (256) each form is also timed in a pool of slots (the batched path: one launch per pass covers every slot), and a
+ * form's score is the geometric mean of its single-solve and pool speeds relative to the reference form.
  *
  * The table is a build input: solvers then pick their forms by N alone (a solve gives the same bits alone and in
- * any pool). NSLAB_TUNE=<file> ("N xpass lp xwaves zfused" per line) overrides it at run time without a rebuild.
+ * any pool). NSLAB_TUNE=<file> ("N xpass lp xwaves zfused pt xs zs" per line) overrides it at run time without a
+ * rebuild.
  *
  *   node js/tune.js [--sizes 32,48,64,96,128,192,256,384,512] [--budget 0.6] [--margin 0.02] [--pool-max 256]
  *                   [--out csrc/nsl_tune_table.inc] [--report tune_report.json] [--dry]
@@ -41,12 +45,14 @@ function args() {
 
 const now = () => Number(process.hrtime.bigint()) / 1e9;
 const key = (c) => c.join(',');
-const describe = (c) => FORM[c[0]] + (c[0] === 1 ? ' x' + c[1] : '') + (c[0] === 4 ? ' ' + (c[2] ? c[2] + '/CU' : 'one group per workgroup') : '') + (c[3] ? ', fused update' : '');
+const describe = (c) => FORM[c[0]] + (c[0] === 1 ? ' x' + c[1] : '') + (c[0] === 4 ? ' ' + (c[2] ? c[2] + '/CU' : 'one group per workgroup') : '') + (c[3] ? ', fused update' : '') +
+  (c[4] ? ', alternative pencil tile' : '') + (c[5] ? ', alternative x-pass schedule' : '') + (c[6] ? ', alternative update schedule' : '');
+const variants = (c) => (c[4] || 0) + (c[5] || 0) + (c[6] || 0);
 
 // Steps/s of the forms `cfg` at N: TGV, nu = 1/1600, fixed dt; three untimed steps (their energy is the
 // same-result check), then about `budget` seconds.
 function timeForm(N, cfg, budget) {
-  addon.tuneSet(N, cfg[0], cfg[1], cfg[2], cfg[3]);
+  addon.tuneSet(N, ...cfg);
   const h = addon.create(N, 0, 1 / 1600, 0.4, 1e-3, 0, 0, 0, 0);
   try {
     const v = addon.run(h, 3, 0, 0);
@@ -76,7 +82,7 @@ const poolSlots = (N) => Math.max(2, Math.min(32, Math.floor(16e9 / addon.poolSl
 // untimed (the step graph is captured on the first launch), then about `budget` seconds of 64-step chunks. The pool's
 // device memory is freed before returning.
 function timePool(N, cfg, slots, budget) {
-  addon.tuneSet(N, cfg[0], cfg[1], cfg[2], cfg[3]);
+  addon.tuneSet(N, ...cfg);
   const p = addon.poolCreate(N, slots);
   try {
     const ids = [];
@@ -107,13 +113,11 @@ function timePool(N, cfg, slots, budget) {
   }
 }
 
-function tuneN(N, a) {
-  const cur = addon.tuneGet(N);
-  const cands = addon.tuneCandidates(N);
+// Times the forms `order` (the reference first) alone and, up to --pool-max, in a pool; each checked against the
+// reference's energy after 3 steps; score = speed relative to the reference.
+function timeRows(N, order, a) {
   const rows = [];
   let ref = null;
-  // the current form first: its energy is the reference for the same-result check
-  const order = [cur].concat(cands.filter((c) => key(c) !== key(cur)));
   for (const c of order) {
     const m = timeForm(N, c, a.budget);
     if (ref === null) ref = m;
@@ -130,10 +134,34 @@ function tuneN(N, a) {
     r.score = r.steps_per_s / rows[0].steps_per_s;
     if (r.pool_solve_steps_per_s) r.score = Math.sqrt(r.score * r.pool_solve_steps_per_s / rows[0].pool_solve_steps_per_s);
   }
+  return rows;
+}
+
+function tuneN(N, a) {
+  const cur = addon.tuneGet(N);
+  const cands = addon.tuneCandidates(N);
+  // 1. x-pass forms and fusion at the default tile and schedules (the current form first: it is the reference)
+  const base = cands.filter((c) => variants(c) === 0 && key(c) !== key(cur));
+  const rows = timeRows(N, [cur].concat(base), a);
+  let pick = choose(rows, a.margin);
+  let gain = pick.score;
+  // 2. tile width and schedules, one at a time from the pick (a variant must beat the pick by the margin)
+  const variantRows = [];
+  for (let round = 0; round < 2; round++) {
+    const p = pick.form;
+    const near = cands.filter((c) => key(c.slice(0, 4)) === key(p.slice(0, 4)) && key(c) !== key(p) &&
+      [4, 5, 6].filter((i) => (c[i] || 0) !== (p[i] || 0)).length === 1 && !variantRows.some((r) => key(r.form) === key(c)));
+    if (!near.length) break;
+    const vr = timeRows(N, [p].concat(near), a);
+    for (const r of vr.slice(1)) variantRows.push(Object.assign(r, { relative_to: describe(p) }));
+    const best = vr.slice(1).reduce((b, r) => (r.score > b.score ? r : b), vr[0]);
+    if (best === vr[0] || best.score < 1 + a.margin) break;
+    gain *= best.score;
+    pick = best;
+  }
   addon.tuneSet(N);   // drop the override
-  const pick = choose(rows, a.margin);
-  return { N, current: describe(cur), pick: pick.form, pick_what: pick.what, kept_current: pick === rows[0],
-    gain: pick.score, candidates: rows };
+  return { N, current: describe(cur), pick: pick.form, pick_what: pick.what, kept_current: key(pick.form) === key(cur),
+    gain, candidates: rows, variants: variantRows };
 }
 
 // The form to keep, from timed rows (the current form first; `score` is the speed relative to it). Forms within the
@@ -153,8 +181,8 @@ function choose(rows, margin) {
 }
 
 function writeTable(file, results) {
-  const lines = ['// {N, xpass, lp, xwaves, zfused}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4).'];
-  for (const r of results) lines.push(`{${r.N}, ${r.pick.join(', ')}},   // ${r.pick_what}`);
+  const lines = ['// {N, xpass, lp, xwaves, zfused, pt, xs, zs}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4).'];
+  for (const r of results) lines.push(`{${r.N}, ${r.pick.concat([0, 0, 0]).slice(0, 7).join(', ')}},   // ${r.pick_what}`);
   fs.writeFileSync(file, lines.join('\n') + '\n');
 }
 
@@ -165,7 +193,7 @@ if (require.main === module) {
     const r = tuneN(N, a);
     results.push(r);
     process.stdout.write(`N=${N}: ${r.pick_what}${r.kept_current ? ' (current kept)' : ''}, x${r.gain.toFixed(3)}; ` +
-      r.candidates.map((c) => `${c.what} ${c.steps_per_s.toFixed(1)}` +
+      r.candidates.concat(r.variants).map((c) => `${c.what} ${c.steps_per_s.toFixed(1)}` +
         (c.pool_solve_steps_per_s ? `/${c.pool_solve_steps_per_s.toFixed(0)} in ${c.pool_slots}` : '')).join(' | ') + '\n');
   }
   if (!a.dry) writeTable(a.out, results);

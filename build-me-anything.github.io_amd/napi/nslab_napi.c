@@ -758,17 +758,17 @@ static napi_value js_pool_release(napi_env env, napi_callback_info info) {
 /* tune forms as [xpass, lp, xwaves, zfused] */
 static napi_value tune_arr(napi_env env, const nsl_tune *t) {
   napi_value arr;
-  const int32_t v4[4] = {t->xpass, t->lp, t->xwaves, t->zfused};
-  CALL(env, napi_create_array_with_length(env, 4, &arr));
-  for (uint32_t i = 0; i < 4; i++) {
+  const int32_t v7[7] = {t->xpass, t->lp, t->xwaves, t->zfused, t->ptile, t->xsched, t->zsched};
+  CALL(env, napi_create_array_with_length(env, 7, &arr));
+  for (uint32_t i = 0; i < 7; i++) {
     napi_value v;
-    CALL(env, napi_create_int32(env, v4[i], &v));
+    CALL(env, napi_create_int32(env, v7[i], &v));
     CALL(env, napi_set_element(env, arr, i, v));
   }
   return arr;
 }
 
-/* tuneGet(N) -> [xpass, lp, xwaves, zfused] */
+/* tuneGet(N) -> [xpass, lp, xwaves, zfused, ptile, xsched, zsched] */
 static napi_value js_tune_get(napi_env env, napi_callback_info info) {
   ARGS(1);
   int64_t N = 0;
@@ -779,31 +779,32 @@ static napi_value js_tune_get(napi_env env, napi_callback_info info) {
   return tune_arr(env, &t);
 }
 
-/* tuneSet(N, xpass, lp, xwaves, zfused), or tuneSet(N) to drop the override */
+/* tuneSet(N, xpass, lp, xwaves, zfused[, ptile, xsched, zsched]), or tuneSet(N) to drop the override */
 static napi_value js_tune_set(napi_env env, napi_callback_info info) {
-  ARGS(5);
-  int64_t a[5] = {0, 0, 0, 0, 0};
-  for (size_t i = 0; i < argc && i < 5; i++) get_i64(env, argv[i], &a[i]);
+  ARGS(8);
+  int64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t i = 0; i < argc && i < 8; i++)
+    if (!get_i64(env, argv[i], &a[i])) a[i] = 0;
   int rc;
   if (argc < 5) {
     rc = nsl_tune_set((int32_t)a[0], NULL);
   } else {
-    nsl_tune t = {(int32_t)a[1], (int32_t)a[2], (int32_t)a[3], (int32_t)a[4]};
+    nsl_tune t = {(int32_t)a[1], (int32_t)a[2], (int32_t)a[3], (int32_t)a[4], (int32_t)a[5], (int32_t)a[6], (int32_t)a[7]};
     rc = nsl_tune_set((int32_t)a[0], &t);
   }
   if (rc < 0) return throw_status(env, rc);
   return NULL;
 }
 
-/* tuneCandidates(N) -> [[xpass, lp, xwaves, zfused], ...] */
+/* tuneCandidates(N) -> [[xpass, lp, xwaves, zfused, ptile, xsched, zsched], ...] */
 static napi_value js_tune_candidates(napi_env env, napi_callback_info info) {
   ARGS(1);
   int64_t N = 0;
   get_i64(env, argv[0], &N);
-  nsl_tune t[128];
-  int k = nsl_tune_candidates((int32_t)N, t, 128);
+  nsl_tune t[512];
+  int k = nsl_tune_candidates((int32_t)N, t, 512);
   if (k < 0) return throw_status(env, k);
-  if (k > 128) k = 128;
+  if (k > 512) k = 512;
   napi_value arr;
   CALL(env, napi_create_array_with_length(env, (size_t)k, &arr));
   for (int i = 0; i < k; i++) {

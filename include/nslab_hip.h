@@ -240,9 +240,14 @@ int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
  *           3 one wave per row group (8x8x8 at 512, 8x4x8 at 256), 4 the same with its input staged through LDS
  *   lp    : row pairs per workgroup of form 1 (1, 2, 4, 8)
  *   xwaves: persistent waves per CU of form 4 (0: one row group per workgroup)
- *   zfused: 1 = z-forward fused with the spectral update (N = 256, 512), 0 = two kernels */
+ *   zfused: 1 = z-forward fused with the spectral update, 0 = two kernels
+ *   ptile : pencil-pass tile width, 0 the default lines per workgroup of N, 1 the alternative (32, 48, 64: 8 lines;
+ *           96: 40; 128, 192: 24; no alternative at other sizes)
+ *   xsched: radix schedule of the staged x pass, 0 the default, 1 the alternative (xpass 4 only)
+ *   zsched: radix schedule of the fused z-forward, 0 the default, 1 the alternative (zfused 1 only) */
 typedef struct {
   int32_t xpass, lp, xwaves, zfused;
+  int32_t ptile, xsched, zsched;
 } nsl_tune;
 /* The forms solvers of this N use: an nsl_tune_set override, else the NSLAB_TUNE file, else the tuned table. */
 int nsl_tune_get(int32_t N, nsl_tune *out);

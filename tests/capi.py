@@ -30,10 +30,11 @@ class RhsDiag(ctypes.Structure):
 
 
 class Tune(ctypes.Structure):
-    _fields_ = [("xpass", ctypes.c_int32), ("lp", ctypes.c_int32), ("xwaves", ctypes.c_int32), ("zfused", ctypes.c_int32)]
+    _fields_ = [("xpass", ctypes.c_int32), ("lp", ctypes.c_int32), ("xwaves", ctypes.c_int32), ("zfused", ctypes.c_int32),
+                ("ptile", ctypes.c_int32), ("xsched", ctypes.c_int32), ("zsched", ctypes.c_int32)]
 
     def astuple(self):
-        return (self.xpass, self.lp, self.xwaves, self.zfused)
+        return (self.xpass, self.lp, self.xwaves, self.zfused, self.ptile, self.xsched, self.zsched)
 
 
 class StepRec(ctypes.Structure):

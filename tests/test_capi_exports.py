@@ -58,7 +58,7 @@ def test_tune_table_and_overrides_without_gpu():
     that size, and dropping an override returns to the table."""
     table = {}
     for line in open(capi.ROOT + "/build-me-anything.github.io_amd/csrc/nsl_tune_table.inc"):
-        m = re.match(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", line)
+        m = re.match(r"\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)\}", line)
         if m and int(m.group(1)) > 0:
             table[int(m.group(1))] = tuple(int(v) for v in m.groups()[1:])
     for N in (16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024):
@@ -67,9 +67,15 @@ def test_tune_table_and_overrides_without_gpu():
         assert cur in cands, (N, cur)
         if N in table:
             assert cur == table[N], N
-    assert len(capi.tune_candidates(256)) == 18 and len(capi.tune_candidates(1024)) == 12
+    # x forms x fusion, crossed with the tile (no alternative at 256, 1024) and the schedule variants (staged x pass,
+    # fused update): 256: (3 pair tiles + wave + 5 staged x 2) + (3 + 1 + 10) x 2 = 42; 1024: rows + 5 staged x 2,
+    # then the same x 2 fused = 11 + 22 = 33; 128 adds the tile: 2 x (4 + 10) + 2 x 2 x (4 + 10) = 84
+    assert len(capi.tune_candidates(256)) == 42 and len(capi.tune_candidates(1024)) == 33
+    assert len(capi.tune_candidates(128)) == 84
     # forms not compiled for a size are refused
-    for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (768, (2, 0, 0, 1)), (512, (1, 1, 0, 0))):
+    for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (768, (2, 0, 0, 1)), (512, (1, 1, 0, 0)),
+                   (256, (4, 0, 16, 1, 1, 0, 0)), (48, (4, 0, 16, 1, 0, 1, 0)), (128, (1, 2, 0, 1, 0, 1, 0)),
+                   (128, (4, 0, 16, 0, 0, 0, 1)), (128, (4, 0, 16, 1, 2, 0, 0))):
         try:
             capi.tune_set(N, bad)
             raise AssertionError("expected an error for %s at %d" % (bad, N))
@@ -81,7 +87,9 @@ def test_tune_table_and_overrides_without_gpu():
     except capi.NslError:
         pass
     capi.tune_set(64, (1, 8, 0, 0))
-    assert capi.tune_get(64) == (1, 8, 0, 0)
+    assert capi.tune_get(64) == (1, 8, 0, 0, 0, 0, 0)
+    capi.tune_set(64, (4, 0, 16, 1, 1, 1, 1))
+    assert capi.tune_get(64) == (4, 0, 16, 1, 1, 1, 1)
     capi.tune_set(64)
     assert capi.tune_get(64) == table[64]
 

@@ -2,8 +2,9 @@
 
 For every size with a size-specialised path from 32 to 512 (row-pair tiles 1, 2, 4, 8, one row per workgroup, one
 wave per row group, the staged wave form with and without persistent waves, each with the z-forward fused into the
-update or not), a random-IC solve (CFL 0.5) run for 3 steps under each form matches the same solve under the table's
-form to 1e-13 relative in the state and the series. The forms differ only in the radix factorisation and tiling of
+update or not, crossed with the pencil passes' alternative tile width and the alternative radix schedules of the
+staged x pass and the fused update), a random-IC solve (CFL 0.5) run for 3 steps under each form matches the same
+solve under the table's form to 1e-13 relative in the state and the series. The forms differ only in the radix factorisation and tiling of
 the same transforms, so the expected difference is roundoff (a few 1e-16). Also: a pool created under an override
 uses it for every slot, and the slot stays bit-identical to a single solve created under the same override."""
 import numpy as np
@@ -31,9 +32,10 @@ def run3(N, cfg):
 def test_every_form_matches_the_tuned_form(N):
     ref_cfg = capi.tune_get(N)
     ref, Eref = run3(N, ref_cfg)
-    forms = capi.tune_candidates(N)
-    if N >= 384:   # the persistent-wave counts share one kernel: check one besides the tuned one
-        forms = [f for f in forms if f[0] != 4 or f[2] in (0, 16)]
+    # the persistent-wave counts share one kernel: check no persistence and 16 per CU besides the tuned form
+    forms = [f for f in capi.tune_candidates(N) if f[0] != 4 or f[2] in (0, 16)]
+    assert any(f[4] for f in forms) == (N in (32, 48, 64, 96, 128, 192)), N
+    assert any(f[5] for f in forms) == (N != 48) and any(f[6] for f in forms), N
     for cfg in forms:
         if cfg == ref_cfg:
             continue
@@ -48,7 +50,7 @@ def test_fused_and_unfused_z_forward_agree(N):
     """The z-forward fused with the update (radix schedules ending in 8: 4*8, 2*3*8, 4*3*8, 16*8, 8*3*8, 16*3*8) and
     the two-kernel form, each against the table's form."""
     ref_cfg = capi.tune_get(N)
-    flip = ref_cfg[:3] + (1 - ref_cfg[3],)
+    flip = ref_cfg[:3] + (1 - ref_cfg[3],) + ref_cfg[4:6] + ((ref_cfg[6] if ref_cfg[3] == 0 else 0),)
     assert flip in capi.tune_candidates(N), (N, flip)
     ref, Eref = run3(N, ref_cfg)
     st, E = run3(N, flip)
@@ -84,12 +86,12 @@ def test_pool_uses_the_override_and_matches_a_single_solve():
 
 
 def test_1024_forms_match_the_tuned_form():
-    """1024^3 (88.6 GB per solve): the row form and the staged form with and without the fused z-forward against
-    the default form."""
+    """1024^3 (88.6 GB per solve): the row form and the staged form with and without the fused z-forward, and the
+    alternative schedules of the staged x pass and the fused update, against the default form."""
     N = 1024
     ref_cfg = capi.tune_get(N)
     ref, Eref = run3(N, ref_cfg)
-    for cfg in ((2, 0, 0, 0), (4, 0, 16, 0), (4, 0, 16, 1)):
+    for cfg in ((2, 0, 0, 0), (4, 0, 16, 0), (4, 0, 16, 1), (4, 0, 16, 1, 0, 1, 0), (4, 0, 16, 1, 0, 0, 1)):
         if cfg == ref_cfg:
             continue
         st, E = run3(N, cfg)

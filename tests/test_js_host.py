@@ -96,13 +96,25 @@ def test_autotuner_runs_and_checks_every_form(tmp_path):
                        capture_output=True, text=True, timeout=600)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+    import capi
     res = json.load(open(rep))["results"]
     assert [x["N"] for x in res] == [32, 256]
     for x in res:
+        N = x["N"]
         forms = [c["form"] for c in x["candidates"]]
-        assert x["pick"] in forms and len(forms) == 18
-        assert all(c["rel_diff_E"] < 1e-11 and c["steps_per_s"] > 0 for c in x["candidates"])
+        # search 1: every x-pass form and fusion at the default tile and schedules (the current form first)
+        base = [list(c) for c in capi.tune_candidates(N) if not any(c[4:])]
+        assert len(forms) == len(base) + (1 if any(forms[0][4:]) else 0)
+        assert all(f in forms for f in base)
+        # search 2: the tile and schedule variants next to the pick (32: tile, both schedules; 256: schedules)
+        vforms = [c["form"] for c in x["variants"]]
+        assert vforms and all(sum(1 for i in (4, 5, 6) if f[i] != r[i]) == 1 for f, r in
+                              ((c["form"], next(p for p in forms + vforms if p[:4] == c["form"][:4] and
+                                                sum(1 for i in (4, 5, 6) if p[i] != c["form"][i]) == 1)) for c in x["variants"]))
+        assert x["pick"] in forms + vforms
+        rows = x["candidates"] + x["variants"]
+        assert all(c["rel_diff_E"] < 1e-11 and c["steps_per_s"] > 0 for c in rows)
         # both sizes are timed in pools too (up to --pool-max 256), and the pick has the best score within 2 %
-        assert all(c["pool_solve_steps_per_s"] > 0 and c["pool_slots"] >= 2 for c in x["candidates"])
+        assert all(c["pool_solve_steps_per_s"] > 0 and c["pool_slots"] >= 2 for c in rows)
         best = max(c["score"] for c in x["candidates"])
         assert x["gain"] * 1.02 >= best
