@@ -376,11 +376,13 @@ static int spectrum_dev(nsl_solver *s, double *out) {
 
 static int spectrum_impl(nsl_solver *s, double *Ek) { return spectrum_dev(s, Ek); }
 
-// cutoffPileUp (nslab.js:557-564): how far the spectrum rises between k8 = round(0.8 kc) and the cutoff kc, as the
-// largest E(k)/E(k8) over that band (>= 1 by construction). Undefined (returns 0) when E(k8) is not above the FP64
-// noise floor of a quadratic quantity, 100 eps^2 times the largest E(k), k >= 1; returns 1 with the value otherwise.
-static int pile_up(int kc, const double *spec, int nb, double *out) {
-  const double eps = 2.220446049250313e-16;   // FP64 machine epsilon
+// cutoffPileUp (nslab.js:557-564; nslab_gpu.py:357-372): how far the spectrum rises between k8 = round(0.8 kc) and
+// the cutoff kc, as the largest E(k)/E(k8) over that band (>= 1 by construction). Undefined (returns 0) when E(k8)
+// is not above the noise floor of a quadratic quantity in the working precision, 100 eps^2 times the largest E(k),
+// k >= 1 (eps: machine epsilon of the arithmetic the spectrum came from); returns 1 with the value otherwise.
+static constexpr double kEps64 = 2.220446049250313e-16;   // FP64 machine epsilon (nslab_gpu.py:64)
+static constexpr double kEps32 = 1.1920929e-07;           // FP32 machine epsilon, as nslab_gpu.py:64 writes it
+static int pile_up(int kc, const double *spec, int nb, double eps, double *out) {
   const int k8 = (int)floor(0.8 * kc + 0.5);
   const double base = spec[k8];
   const double peak = nb > 1 ? *std::max_element(spec + 1, spec + nb) : 0.0;
@@ -648,7 +650,7 @@ static int diagnose_impl(nsl_solver *s, int do_interp, double *out) {
   out[17] = -(6 * sqrt(15.0) / 7) * d.Pspec / pow(std::max(2 * d.Z, 1e-300), 1.5);
   for (int e = 0; e < 3; e++) out[18 + e] = hs[8 + e] / cn;
   out[21] = pow(u3 / (double)N3, 1.0 / 3);
-  { double pu; out[22] = pile_up(N / 3, spec, nbins, &pu) ? pu : NAN; }
+  { double pu; out[22] = pile_up(N / 3, spec, nbins, s->fp32 ? kEps32 : kEps64, &pu) ? pu : NAN; }
   image_diag(ez, N, out + 23);
   if (do_interp && N <= 128) TRY(interp_max_impl(s, 24, 12, out + 27));
   else for (int q = 27; q < 32; q++) out[q] = NAN;

@@ -87,8 +87,8 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restric
 template <int N> struct ZinvMinWaves { static constexpr int v = N > 512 ? NSL_ZINV_BIG_MINW : N == 512 ? 4 : 1; };
 // Threads of the z-inverse: at N = 256 (16 x 16, 16 outputs per butterfly) 128 threads, one butterfly each.
 template <int N> struct ZinvNT { static constexpr int v = N == 256 ? 128 : PassNT<N>::v; };
-template <int N, int T>
-__global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(const double2 *__restrict__ U, double2 *__restrict__ X, const Geo,
+template <int N, int T, class E = double2>
+__global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(const E *__restrict__ U, E *__restrict__ X, const Geo,
                                                       const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
   U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
   const int tile = bid % ntile, rest = bid / ntile, type = rest & 1, cy = rest >> 1, kx0 = tile * T;
   const double ky = kwv(full_of(cy, g), N);
   const size_t ubase = (size_t)cy * g.hp + kx0;
-  double2 *Xb = X + (size_t)cy * g.hp + kx0;
+  E *Xb = X + (size_t)cy * g.hp + kx0;
   OR A, B;
   auto keepA = [&](int, int, double2 v, int k, int j) { A.v[k][j] = v; };
   auto keepB = [&](int, int, double2 v, int k, int j) { B.v[k][j] = v; };
@@ -114,13 +114,13 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
       const int cz = compact_of(n, g);
       if constexpr (N >= 512) {
         if (cz < 0 || kx0 + t >= g.hx) return make_double2(0.0, 0.0);
-        const double2 u = U[(size_t)c * g.D + ubase + (size_t)cz * plane + t];
+        const double2 u = ldc(U[(size_t)c * g.D + ubase + (size_t)cz * plane + t]);
         if (!ikz) return u;
         const double kz = kwv(n, N);
         return make_double2(-kz * u.y, kz * u.x);
       } else {
         if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
-        const double2 u = (cz >= 0 && kx0 + t < g.hx) ? U[(size_t)c * g.D + ubase + (size_t)cz * plane + t]
+        const double2 u = (cz >= 0 && kx0 + t < g.hx) ? ldc(U[(size_t)c * g.D + ubase + (size_t)cz * plane + t])
                                                       : make_double2(0.0, 0.0);
         const double kz = kwv(n, N);
         return ikz ? make_double2(-kz * u.y, kz * u.x) : u;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
     for (int k = 0; k < OR::K; k++) {
       if (!OR::valid(k) || (!NSL_FULL_LINES && kx0 + OR::line(k) >= g.hx)) continue;
 #pragma unroll
-      for (int j = 0; j < OR::R; j++) Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)] = f(k, j);
+      for (int j = 0; j < OR::R; j++) stc(Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)], f(k, j));
     }
   };
   if (type == 0) {
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
 }
 
 // P2: X1[o][z][cy][kx] -> X2[o][z][y][kx]. Grid (ntile*N, ncomp).
-template <int N, int T>
-__global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restrict__ X1, double2 *__restrict__ X2, const Geo,
+template <int N, int T, class E = double2>
+__global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const E *__restrict__ X1, E *__restrict__ X2, const Geo,
                                                    const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
   X1 = at_slot(X1, sb); X2 = at_slot(X2, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
@@ -177,24 +177,24 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const double2 *__restric
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
-  const double2 *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
-  double2 *Ob = X2 + ((size_t)o * N + z) * N * g.hp + kx0;
+  const E *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
+  E *Ob = X2 + ((size_t)o * N + z) * N * g.hp + kx0;
   auto load = [&](int n, int t, int, int j) -> double2 {
     if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
     const int cy = compact_of(n, g);
-    return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : make_double2(0.0, 0.0);
+    return (cy >= 0 && kx0 + t < g.hx) ? ldc(Ib[cy * g.hp + t]) : make_double2(0.0, 0.0);
   };
   // every column of the tile is stored, padding kx >= hx included (its inputs are zero, so it stores zeros): whole
   // 128-byte lines, never a partial line
   auto store = [&](int n, int t, double2 v) {
-    if (NSL_FULL_LINES || kx0 + t < g.hx) Ob[n * g.hp + t] = v;
+    if (NSL_FULL_LINES || kx0 + t < g.hx) stc(Ob[n * g.hp + t], v);
   };
   fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
 }
 
 // P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
-template <int N, int T>
-__global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restrict__ X3, double2 *__restrict__ X4, const Geo,
+template <int N, int T, class E = double2>
+__global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const E *__restrict__ X3, E *__restrict__ X4, const Geo,
                                                    const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb,
                                                    const double *__restrict__ pmax, int nblk, int nred) {
   X3 = at_slot(X3, sb); X4 = at_slot(X4, sb); ctl = at_slot(ctl, sb);
@@ -219,14 +219,14 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const double2 *__restric
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
-  const double2 *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
-  double2 *Ob = X4 + ((size_t)c * N + z) * g.by * g.hp + kx0;
+  const E *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
+  E *Ob = X4 + ((size_t)c * N + z) * g.by * g.hp + kx0;
   auto load = [&](int n, int t) -> double2 {
-    return kx0 + t < g.hx ? Ib[n * g.hp + t] : make_double2(0.0, 0.0);
+    return kx0 + t < g.hx ? ldc(Ib[n * g.hp + t]) : make_double2(0.0, 0.0);
   };
   auto store = [&](int n, int t, double2 v) {   // whole tiles, as in the y-inverse
     const int cy = compact_of(n, g);
-    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx)) Ob[cy * g.hp + t] = v;
+    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx)) stc(Ob[cy * g.hp + t], v);
   };
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
@@ -319,9 +319,9 @@ template <int N> struct ZuSlots {
     return 0;
   }
 };
-template <int N, int STAGE, int V = 0>
-__global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(const double2 *__restrict__ X4, const double2 *U,
-                                                            double2 *S, double2 *__restrict__ ACC, double2 *TS,
+template <int N, int STAGE, int V = 0, class E = double2>
+__global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(const E *__restrict__ X4, const E *U,
+                                                            E *S, E *__restrict__ ACC, E *TS,
                                                             const Geo, const double2 *__restrict__ tw, double inv,
                                                             double *__restrict__ partials, Ctl *__restrict__ ctl,
                                                             size_t sb) {
@@ -335,12 +335,12 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
   const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, kx0 = tile * T;
-  const double2 *Ib = X4 + (size_t)cy * g.hp + kx0;
+  const E *Ib = X4 + (size_t)cy * g.hp + kx0;
   double2 Nl[3][R];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     auto load = [&](int n, int t) -> double2 {
-      return kx0 + t < g.hx ? Ib[((size_t)c * N + n) * plane + t] : make_double2(0.0, 0.0);
+      return kx0 + t < g.hx ? ldc(Ib[((size_t)c * N + n) * plane + t]) : make_double2(0.0, 0.0);
     };
     auto keep = [&](int, int, double2 v, int, int j) { Nl[c][j] = make_double2(v.x * inv, v.y * inv); };
     fft_sched<N, T, NT, -1, false, Interleaved<N, T>, typename ZuSchedV<N, V>::L>(lds, tw, load, keep);
@@ -388,8 +388,8 @@ template <int N, int T>
 using XnlLayout = typename std::conditional<NSL_XNL_LINEMAJOR != 0, LineMajor<N, T>, Interleaved<N, T>>::type;
 
 // P3: in place on X2 (6 fields in, 3 out), LP row pairs per workgroup. Same packing and product as k_xnl.
-template <int N, int LP>
-__global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+template <int N, int LP, class E = double2>
+__global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
@@ -418,8 +418,8 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
         if (pr < npairs) {
           const int z = pr / half, y = 2 * (pr - z * half);
           const size_t base = o * cs + ((size_t)z * N + y) * HP;
-          A[i] = X2[base + k];
-          B[i] = X2[base + HP + k];
+          A[i] = ldc(X2[base + k]);
+          B[i] = ldc(X2[base + HP + k]);
         }
       }
     }
@@ -505,8 +505,8 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
       Y = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
     }
     const size_t base = c * cs + ((size_t)z * N + y) * HP;
-    X2[base + k] = X;
-    X2[base + HP + k] = Y;
+    stc(X2[base + k], X);
+    stc(X2[base + HP + k], Y);
   }
   }
   block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds);
@@ -515,8 +515,8 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(double2 *__restrict__ X2, const
 // P3 with RP rows per workgroup of NT threads (half the LDS of the row-pair form for RP = 1). Each row's six
 // half-spectra ride as three complex lines, (u_x + i u_y), (u_z + i om_x), (om_y + i om_z); after the inverse the
 // product u x omega is formed per point, and (n_x + i n_y), (n_z + i 0) go through the forward transform.
-template <int N, int RP, int NT>
-__global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+template <int N, int RP, int NT, class E = double2>
+__global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
                                                    double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
@@ -538,8 +538,8 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
       const int t = e / hx, k = e - t * hx, r = t / 3, l = t - r * 3;   // line l of row r: fields 2l, 2l+1
       if (row0 + r < nrows) {
         const size_t base = (size_t)(row0 + r) * HP + k;
-        A[i] = X2[2 * l * cs + base];
-        B[i] = X2[(2 * l + 1) * cs + base];
+        A[i] = ldc(X2[2 * l * cs + base]);
+        B[i] = ldc(X2[(2 * l + 1) * cs + base]);
       }
     }
   }
@@ -614,9 +614,9 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(double2 *__restrict__ X2, cons
         o2 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
       }
     }
-    X2[base] = o0;
-    X2[cs + base] = o1;
-    X2[2 * cs + base] = o2;
+    stc(X2[base], o0);
+    stc(X2[cs + base], o1);
+    stc(X2[2 * cs + base], o2);
   }
   // per-workgroup maxima
   om2 = wave_max(om2); u2 = wave_max(u2);
@@ -653,8 +653,8 @@ template <> struct XnlWave<192> { using L = RList<8, 3, 8>; };
 template <> struct XnlWave<384> { using L = RList<8, 6, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
 template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
-template <int N>
-__global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+template <int N, class E = double2>
+__global__ __launch_bounds__(64) void kf_xnl_wave(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
   constexpr int R = 8, NB = N / R, RPW = 64 / NB;   // butterflies per line and stage ends, rows per wave
   static_assert(NB * RPW == 64, "first and last stage must cover a wave");
@@ -681,10 +681,10 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
     auto ld = [&](int n, int t, int, int j) -> double2 {
       if (j * NB >= hx && j * NB + NB - 1 <= N - hx) return make_double2(0.0, 0.0);
       const size_t base = (size_t)(row0 + t) * HP;
-      const double2 *Fa = X2 + 2 * l * cs + base, *Fb = X2 + (2 * l + 1) * cs + base;
+      const E *Fa = X2 + 2 * l * cs + base, *Fb = X2 + (2 * l + 1) * cs + base;
       const bool lo = n < hx, hi = n > N - hx;
       const int m = lo ? n : (hi ? N - n : 0);
-      const double2 A = Fa[m], B = Fb[m];
+      const double2 A = ldc(Fa[m]), B = ldc(Fb[m]);
       const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
       return lo || hi ? d : make_double2(0.0, 0.0);
     };
@@ -729,10 +729,10 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(double2 *__restrict__ X2, cons
         }
       }
       if (l == 0) {
-        X2[base + k] = o0;
-        X2[cs + base + k] = o1;
+        stc(X2[base + k], o0);
+        stc(X2[cs + base + k], o1);
       } else {
-        X2[2 * cs + base + k] = o0;
+        stc(X2[2 * cs + base + k], o0);
       }
     }
     __syncthreads();   // the next line's first stage overwrites LDS
@@ -904,10 +904,10 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
         }
       }
       if (l == 0) {
-        X2[base + k] = o0;
-        X2[cs + base + k] = o1;
+        stc(X2[base + k], o0);
+        stc(X2[cs + base + k], o1);
       } else {
-        X2[2 * cs + base + k] = o0;
+        stc(X2[2 * cs + base + k], o0);
       }
     }
     stage_sync<true, NT>();   // the next line's first stage overwrites xch

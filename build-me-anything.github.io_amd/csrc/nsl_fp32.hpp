@@ -1,0 +1,137 @@
+// Passes of an FP32 solve: the spectral arrays in complex64 (float2) in HBM, the arithmetic in registers in FP64.
+// The same size-specialised rhs kernels (nsl_fastk.hpp) instantiated with storage E = float2, at the pencil tile
+// fast_T(N), with the z-forward fused into the update and one x-pass form per N. Included by nslab_hip.hip after
+// nsl_tune.hpp. An FP32 solve is a single solve (never a pool slot), so every launch has gridDim.z = 1 and sb = 0.
+//
+// The reference's FP32 mode (nslab_gpu.py:56-65) holds every array in complex64/float32 and transforms in single
+// precision. Here every array that lives in HBM between passes is complex64, so the state, the RK4 registers and
+// each pass's output carry FP32 rounding as there; the butterflies, the product u x omega and the update arithmetic
+// run in FP64 registers, which costs nothing on a path bound by HBM and gives a little less roundoff than an
+// all-single pipeline.
+#pragma once
+
+// Sizes with an FP32 path: those with the fused z-forward and update.
+static constexpr bool fp32_has(int N) { return zu_has(N); }
+// x-pass form of FP32 solves: one wave per row group at 256^3 and 512^3, one row per workgroup above 384^3, row
+// pairs (fast_LP(N) per workgroup) otherwise.
+static constexpr int xp32_form(int N) {
+  return (N == 256 || N == 512) ? NSL_XP_WAVE : N > 384 ? NSL_XP_ROWS : NSL_XP_PAIR;
+}
+// Workgroups of the FP32 x pass (one maxima pair each, at most N^2).
+static constexpr long long xnl32_workgroups(int N) {
+  return xp32_form(N) == NSL_XP_WAVE ? (long long)N * N / (64 / (N / 8))
+       : xp32_form(N) == NSL_XP_ROWS ? (long long)N * N
+       : ((long long)N * (N / 2) + fast_LP(N) - 1) / fast_LP(N);
+}
+
+template <int N> struct P32 {
+  static constexpr int T = fast_T(N);
+  static constexpr Geo g = fast::trunc_geo<N>();
+  static constexpr int ntile = (g.hx + T - 1) / T;
+  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
+
+  // P1: u and omega of U along z (six fields into X1)
+  static int zinv(const Xform &x, const float2 *U, float2 *X1, const Ctl *ctl) {
+    const int lds = N * T * (int)sizeof(double2);
+    allow_lds(fast::kf_zinv_uw<N, T, float2>, lds);
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T, float2>), dim3((unsigned)(ntile * 2 * g.by)), dim3(fast::ZinvNT<N>::v), lds,
+                       x.st, U, X1, x.g, x.twN, ctl, (size_t)0);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  // P2: six fields along y
+  static int yinv(const Xform &x, const float2 *X1, float2 *X2, const Ctl *ctl) {
+    const int lds = N * T * (int)sizeof(double2);
+    allow_lds(fast::kf_yinv<N, T, float2>, lds);
+    hipLaunchKernelGGL((fast::kf_yinv<N, T, float2>), dim3((unsigned)(ntile * N), 6u), dim3(fast::PassNT<N>::v), lds, x.st,
+                       X1, X2, x.g, x.twN, ctl, (size_t)0);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  // P3: x C2R, u x omega, x R2C in place on X2; one maxima pair per workgroup into pmax (N^2 pairs)
+  static int xnl(const Xform &x, float2 *X2, double *pmax, Ctl *ctl, int *nred) {
+    constexpr long long grid = xnl32_workgroups(N);
+    static_assert(grid > 0 && grid <= (long long)N * N, "x-pass maxima fit their buffer");
+    if constexpr (xp32_form(N) == NSL_XP_WAVE) {
+      constexpr int RPW = 64 / (N / 8);
+      const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
+      hipLaunchKernelGGL((fast::kf_xnl_wave<N, float2>), dim3((unsigned)grid), dim3(64), lds, x.st, X2, x.g, x.twN, pmax,
+                         ctl, (size_t)0);
+    } else if constexpr (xp32_form(N) == NSL_XP_ROWS) {
+      constexpr int RP = 1, NT = 192;
+      const int lds = N * 3 * RP * (int)sizeof(double2);
+      allow_lds(fast::kf_xnl_rows<N, RP, NT, float2>, lds);
+      hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT, float2>), dim3((unsigned)grid), dim3(NT), lds, x.st, X2, x.g, x.twN,
+                         pmax, ctl, (size_t)0);
+    } else {
+      constexpr int LP = fast_LP(N);
+      static_assert(xp_pair_lp_ok(N, LP), "row pairs fit LDS");
+      const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
+      allow_lds(fast::kf_xnl<N, LP, float2>, lds);
+      hipLaunchKernelGGL((fast::kf_xnl<N, LP, float2>), dim3((unsigned)grid), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax,
+                         ctl, (size_t)0);
+    }
+    LAUNCH_CHECK();
+    *nred = (int)grid;
+    return NSL_OK;
+  }
+  // P4: the three nonlinear components along y, truncated to the retained ky; combines the x pass's maxima
+  static int yfwd(const Xform &x, const float2 *X2, float2 *X1, const Ctl *ctl, const double *pmax, int nblk) {
+    const int lds = N * T * (int)sizeof(double2);
+    allow_lds(fast::kf_yfwd<N, T, float2>, lds);
+    const unsigned grid = (unsigned)(ntile * N);
+    const int nred = pmax && nblk > 0 ? (int)std::min<long long>(grid, nblk / fast::PassNT<N>::v + 1) : 0;
+    hipLaunchKernelGGL((fast::kf_yfwd<N, T, float2>), dim3(grid, 3u), dim3(fast::PassNT<N>::v), lds, x.st, X2, X1, x.g,
+                       x.twN, ctl, (size_t)0, pmax, nblk, nred);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  // P5: z-forward fused with the stage-`stage` update
+  template <int STAGE>
+  static void zu_stage(const Xform &x, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
+                       double *partials, Ctl *ctl) {
+    using C = fast::ZuCfg<N>;
+    const int lds = N * C::T * (int)sizeof(double2);
+    const double inv = 1.0 / ((double)N * N * N);
+    allow_lds(fast::kf_zfwd_upd<N, STAGE, 0, float2>, lds);
+    hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE, 0, float2>), dim3((unsigned)fast_zu_blocks(x.g)), dim3(C::NT), lds,
+                       x.st, X1, U, S, ACC, TS, x.g, x.twN, inv, partials, ctl, (size_t)0);
+  }
+  static int zu(const Xform &x, int stage, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
+                double *partials, Ctl *ctl) {
+    switch (stage) {
+      case 1: zu_stage<1>(x, X1, U, S, ACC, TS, partials, ctl); break;
+      case 2: zu_stage<2>(x, X1, U, S, ACC, TS, partials, ctl); break;
+      case 3: zu_stage<3>(x, X1, U, S, ACC, TS, partials, ctl); break;
+      default: zu_stage<4>(x, X1, U, S, ACC, TS, partials, ctl); break;
+    }
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+};
+
+#define NSL_FP32_DISPATCH(call)                                                   \
+  switch (x.g.N) {                                                                \
+    case 32: return P32<32>::call;                                                \
+    case 48: return P32<48>::call;                                                \
+    case 64: return P32<64>::call;                                                \
+    case 96: return P32<96>::call;                                                \
+    case 128: return P32<128>::call;                                              \
+    case 192: return P32<192>::call;                                              \
+    case 256: return P32<256>::call;                                              \
+    case 384: return P32<384>::call;                                              \
+    case 512: return P32<512>::call;                                              \
+    case 1024: return P32<1024>::call;                                            \
+    default: return fail(NSL_ERR_STATE, "no FP32 kernels for N = " + std::to_string(x.g.N)); \
+  }
+
+static int zinv32(const Xform &x, const float2 *U, float2 *X1, const Ctl *ctl) { NSL_FP32_DISPATCH(zinv(x, U, X1, ctl)) }
+static int yinv32(const Xform &x, const float2 *X1, float2 *X2, const Ctl *ctl) { NSL_FP32_DISPATCH(yinv(x, X1, X2, ctl)) }
+static int xnl32(const Xform &x, float2 *X2, double *pmax, Ctl *ctl, int *nred) { NSL_FP32_DISPATCH(xnl(x, X2, pmax, ctl, nred)) }
+static int yfwd32(const Xform &x, const float2 *X2, float2 *X1, const Ctl *ctl, const double *pmax, int nblk) {
+  NSL_FP32_DISPATCH(yfwd(x, X2, X1, ctl, pmax, nblk))
+}
+static int zu32(const Xform &x, int stage, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
+                double *partials, Ctl *ctl) {
+  NSL_FP32_DISPATCH(zu(x, stage, X1, U, S, ACC, TS, partials, ctl))
+}

@@ -46,6 +46,18 @@ struct StepRecDev {
   double d0[8];
 };
 
+// Storage element of the spectral arrays: double2 (FP64 solves) or float2 (FP32 solves: complex64 in HBM, the
+// arithmetic in registers stays FP64). ldc widens a stored value, stc rounds a value to the storage precision, and
+// rnd<E> is the value a store to E followed by a load gives back (the identity for double2).
+__device__ __forceinline__ double2 ldc(const double2 &v) { return v; }
+__device__ __forceinline__ double2 ldc(const float2 &v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ void stc(double2 &d, const double2 &v) { d = v; }
+__device__ __forceinline__ void stc(float2 &d, const double2 &v) { d = make_float2((float)v.x, (float)v.y); }
+template <class E> __device__ __forceinline__ double2 rnd(const double2 &v) {
+  if constexpr (std::is_same<E, double2>::value) return v;
+  else return make_double2((double)(float)v.x, (double)(float)v.y);
+}
+
 // chooseDt (nslab.js:305-310), evaluated with the reference's operation order.
 __device__ __forceinline__ double choose_dt(const Ctl *c, double uMaxIn) {
   if (c->dtFixed > 0) return c->dtFixed;
@@ -366,22 +378,21 @@ __device__ __forceinline__ bool mode_k(long long m, const Geo &g, double &kx, do
 struct UpdIn {
   double2 u[3], s[3], a[3];
 };
-template <int STAGE>
-__device__ __forceinline__ UpdIn upd_load(size_t m, long long D, const double2 *U, const double2 *S,
-                                          const double2 *ACC) {
+template <int STAGE, class E>
+__device__ __forceinline__ UpdIn upd_load(size_t m, long long D, const E *U, const E *S, const E *ACC) {
   UpdIn r;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    r.u[c] = U[c * D + m];
-    r.s[c] = STAGE == 1 ? r.u[c] : S[c * D + m];   // stage 1 evaluates the rhs at S itself
-    r.a[c] = STAGE == 1 ? make_double2(0.0, 0.0) : ACC[c * D + m];
+    r.u[c] = ldc(U[c * D + m]);
+    r.s[c] = STAGE == 1 ? r.u[c] : ldc(S[c * D + m]);   // stage 1 evaluates the rhs at S itself
+    r.a[c] = STAGE == 1 ? make_double2(0.0, 0.0) : ldc(ACC[c * D + m]);
   }
   return r;
 }
-template <int STAGE>
+template <int STAGE, class E>
 __device__ __forceinline__ void upd_apply(const UpdIn &in, size_t m, long long D, double kx, double ky, double kz,
-                                          double h, double2 n0, double2 n1, double2 n2, double2 *S, double2 *ACC,
-                                          double2 *TS, double nu, double dt, double (&acc)[8]) {
+                                          double h, double2 n0, double2 n1, double2 n2, E *S, E *ACC, E *TS,
+                                          double nu, double dt, double (&acc)[8]) {
   const double k2 = kx * kx + ky * ky + kz * kz;
   if (k2 == 0) {
     n0 = n1 = n2 = make_double2(0.0, 0.0);
@@ -402,24 +413,25 @@ __device__ __forceinline__ void upd_apply(const UpdIn &in, size_t m, long long D
   const double2 s0 = in.s[0], s1 = in.s[1], s2 = in.s[2];
   if (STAGE == 1 || STAGE == 2 || STAGE == 3) {
     const double f = STAGE == 3 ? dt : 0.5 * dt;
-    TS[m] = make_double2(s0.x + f * K0.x, s0.y + f * K0.y);
-    TS[D + m] = make_double2(s1.x + f * K1.x, s1.y + f * K1.y);
-    TS[2 * D + m] = make_double2(s2.x + f * K2.x, s2.y + f * K2.y);
+    stc(TS[m], make_double2(s0.x + f * K0.x, s0.y + f * K0.y));
+    stc(TS[D + m], make_double2(s1.x + f * K1.x, s1.y + f * K1.y));
+    stc(TS[2 * D + m], make_double2(s2.x + f * K2.x, s2.y + f * K2.y));
     if (STAGE == 1) {
-      ACC[m] = K0; ACC[D + m] = K1; ACC[2 * D + m] = K2;
+      stc(ACC[m], K0); stc(ACC[D + m], K1); stc(ACC[2 * D + m], K2);
     } else {
       const double2 a0 = in.a[0], a1 = in.a[1], a2 = in.a[2];
-      ACC[m] = make_double2(a0.x + 2 * K0.x, a0.y + 2 * K0.y);
-      ACC[D + m] = make_double2(a1.x + 2 * K1.x, a1.y + 2 * K1.y);
-      ACC[2 * D + m] = make_double2(a2.x + 2 * K2.x, a2.y + 2 * K2.y);
+      stc(ACC[m], make_double2(a0.x + 2 * K0.x, a0.y + 2 * K0.y));
+      stc(ACC[D + m], make_double2(a1.x + 2 * K1.x, a1.y + 2 * K1.y));
+      stc(ACC[2 * D + m], make_double2(a2.x + 2 * K2.x, a2.y + 2 * K2.y));
     }
   } else {
     const double f = dt / 6;
     const double2 a0 = in.a[0], a1 = in.a[1], a2 = in.a[2];
-    const double2 r0 = make_double2(s0.x + f * (a0.x + K0.x), s0.y + f * (a0.y + K0.y));
-    const double2 r1 = make_double2(s1.x + f * (a1.x + K1.x), s1.y + f * (a1.y + K1.y));
-    const double2 r2 = make_double2(s2.x + f * (a2.x + K2.x), s2.y + f * (a2.y + K2.y));
-    S[m] = r0; S[D + m] = r1; S[2 * D + m] = r2;
+    // the new state as stored (rounded to the storage precision), so its E, Z, P match the state itself
+    const double2 r0 = rnd<E>(make_double2(s0.x + f * (a0.x + K0.x), s0.y + f * (a0.y + K0.y)));
+    const double2 r1 = rnd<E>(make_double2(s1.x + f * (a1.x + K1.x), s1.y + f * (a1.y + K1.y)));
+    const double2 r2 = rnd<E>(make_double2(s2.x + f * (a2.x + K2.x), s2.y + f * (a2.y + K2.y)));
+    stc(S[m], r0); stc(S[D + m], r1); stc(S[2 * D + m], r2);
     const double en = h * (r0.x * r0.x + r0.y * r0.y + r1.x * r1.x + r1.y * r1.y + r2.x * r2.x + r2.y * r2.y);
     acc[5] += en; acc[6] += k2 * en; acc[7] += k2 * k2 * en;
   }
@@ -427,16 +439,16 @@ __device__ __forceinline__ void upd_apply(const UpdIn &in, size_t m, long long D
 
 // Zeros where upd_apply<STAGE> writes, for a padding column kx >= hx of the compact rows (its caller writes whole
 // 128-byte lines).
-template <int STAGE>
-__device__ __forceinline__ void upd_zero(size_t m, long long D, double2 *S, double2 *ACC, double2 *TS) {
+template <int STAGE, class E>
+__device__ __forceinline__ void upd_zero(size_t m, long long D, E *S, E *ACC, E *TS) {
   const double2 z = make_double2(0.0, 0.0);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     if (STAGE == 4) {
-      S[c * D + m] = z;
+      stc(S[c * D + m], z);
     } else {
-      TS[c * D + m] = z;
-      ACC[c * D + m] = z;
+      stc(TS[c * D + m], z);
+      stc(ACC[c * D + m], z);
     }
   }
 }

@@ -325,6 +325,7 @@ static void xform_free(Xform &x) {
 
 #include "nsl_fastdisp.hpp"
 #include "nsl_tune.hpp"
+#include "nsl_fp32.hpp"
 
 // ------------------------------------------------------------------------------------------------ solver
 #define NSL_RECCAP 64
@@ -334,7 +335,9 @@ struct SlotLayout {
   size_t S, T, ACC, NL, X1, X2, partials, pmax, ctl, recs, bytes;
 };
 
-static SlotLayout slot_layout(const Geo &g, int nblk) {
+// esz: bytes per stored complex value (16 for FP64 solves, 8 for FP32); an FP32 solve keeps no NL (its update is
+// always the fused z-forward, which holds the nonlinear term on chip).
+static SlotLayout slot_layout(const Geo &g, int nblk, size_t esz = sizeof(double2), bool with_nl = true) {
   SlotLayout L;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -342,10 +345,10 @@ static SlotLayout slot_layout(const Geo &g, int nblk) {
     off += (bytes + 255) & ~(size_t)255;
     return o;
   };
-  const size_t D3 = (size_t)3 * g.D * sizeof(double2);
-  L.S = take(D3); L.T = take(D3); L.ACC = take(D3); L.NL = take(D3);
-  L.X1 = take((size_t)6 * g.N * g.by * g.hp * sizeof(double2));
-  L.X2 = take((size_t)6 * g.N * g.N * g.hp * sizeof(double2));
+  const size_t D3 = (size_t)3 * g.D * esz;
+  L.S = take(D3); L.T = take(D3); L.ACC = take(D3); L.NL = with_nl ? take(D3) : 0;
+  L.X1 = take((size_t)6 * g.N * g.by * g.hp * esz);
+  L.X2 = take((size_t)6 * g.N * g.N * g.hp * esz);
   // per-block sums of the spectral sweep (nblk blocks) or of the fused z-forward (at most by * ceil(hx / 4))
   L.partials = take((size_t)std::max(nblk, g.by * ((g.hx + 3) / 4)) * 8 * sizeof(double));
   L.pmax = take((size_t)2 * g.N * g.N * sizeof(double));   // one (max |omega|^2, max |u|^2) per x-pass workgroup
@@ -385,6 +388,11 @@ struct nsl_solver {
   double *dpart = nullptr;  // diagnose partials
   int diag_valid = 0;
   hipEvent_t evfence = nullptr;   // a pool's launch object: orders a group launch after the pool's own stream
+  // FP32 solve (nsl_create_fp32): the stepped buffers in complex64 (block); S..X2 above then point into `aux`, an
+  // FP64 set that exists only while a host view (initial field, state in or out, diagnostics) runs
+  int fp32 = 0;
+  float2 *S32 = nullptr, *T32 = nullptr, *ACC32 = nullptr, *X1_32 = nullptr, *X2_32 = nullptr;
+  char *aux = nullptr;
   // optional per-kernel timing with HIP events on the solver's stream (nsl_profile)
   int prof = 0;
   std::vector<hipEvent_t> evs;
@@ -437,8 +445,13 @@ static int dalloc(nsl_solver *s, T **p, size_t count) {
 
 static void bind_slot(nsl_solver *s, char *base, const SlotLayout &L) {
   s->block = base;
-  s->S = (double2 *)(base + L.S); s->T = (double2 *)(base + L.T); s->ACC = (double2 *)(base + L.ACC);
-  s->NL = (double2 *)(base + L.NL); s->X1 = (double2 *)(base + L.X1); s->X2 = (double2 *)(base + L.X2);
+  if (s->fp32) {
+    s->S32 = (float2 *)(base + L.S); s->T32 = (float2 *)(base + L.T); s->ACC32 = (float2 *)(base + L.ACC);
+    s->X1_32 = (float2 *)(base + L.X1); s->X2_32 = (float2 *)(base + L.X2);
+  } else {
+    s->S = (double2 *)(base + L.S); s->T = (double2 *)(base + L.T); s->ACC = (double2 *)(base + L.ACC);
+    s->NL = (double2 *)(base + L.NL); s->X1 = (double2 *)(base + L.X1); s->X2 = (double2 *)(base + L.X2);
+  }
   s->partials = (double *)(base + L.partials); s->pmax = (double *)(base + L.pmax);
   s->ctl = (Ctl *)(base + L.ctl); s->recs = (StepRecDev *)(base + L.recs);
 }
@@ -459,10 +472,31 @@ static void diag_from(const double *d, nsl_rhsdiag &o) {
   o.E = d[0]; o.Z = d[1]; o.P = d[2]; o.eps = d[3]; o.Tnl = d[4]; o.Pspec = d[5]; o.omMax = d[6]; o.uMax = d[7];
 }
 
-// One rhs of U with the fused stage-`stage` update; fmode selects where k_finalize books the sums.
-static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecDev *rec) {
+// One rhs of an FP32 solve's S32 (onS) or T32, with the fused stage-`stage` update.
+static int rhs_enqueue32(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev *rec) {
+  const Xform &x = s->x;
+  const float2 *U = onS ? s->S32 : s->T32;
+  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv32(x, U, s->X1_32, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv32(x, s->X1_32, s->X2_32, s->ctl)); TRY(prof_end(s));
+  int nred = 0;
+  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl32(x, s->X2_32, s->pmax, s->ctl, &nred)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd32(x, s->X2_32, s->X1_32, s->ctl, s->pmax, nred)); TRY(prof_end(s));
+  TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
+  TRY(zu32(x, stage, s->X1_32, U, s->S32, s->ACC32, s->T32, s->partials, s->ctl));
+  TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_FINALIZE));
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSL_NT), 0, s->st, s->partials, fast_zu_blocks(x.g), fmode, rec, s->ctl, (size_t)0);
+  LAUNCH_CHECK();
+  TRY(prof_end(s));
+  return NSL_OK;
+}
+
+// One rhs of S (onS) or T with the fused stage-`stage` update; fmode selects where k_finalize books the sums.
+static int rhs_enqueue(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev *rec) {
+  if (s->fp32) return rhs_enqueue32(s, onS, stage, fmode, rec);
   const Xform &x = s->x;
   const Batch &bt = s->bt;
+  double2 *U = onS ? s->S : s->T;
   TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
   int nred = 0;
@@ -500,10 +534,10 @@ static int rhs_enqueue(nsl_solver *s, double2 *U, int stage, int fmode, StepRecD
 static int step_enqueue(nsl_solver *s) {
   hipLaunchKernelGGL(k_begin_step, dim3(1, 1, s->bt.nz), dim3(1), 0, s->st, s->ctl, s->bt.sb);
   LAUNCH_CHECK();
-  TRY(rhs_enqueue(s, s->T, 2, 1, nullptr));
-  TRY(rhs_enqueue(s, s->T, 3, 2, nullptr));
-  TRY(rhs_enqueue(s, s->T, 4, 3, nullptr));
-  TRY(rhs_enqueue(s, s->S, 1, 4, s->recs));
+  TRY(rhs_enqueue(s, false, 2, 1, nullptr));
+  TRY(rhs_enqueue(s, false, 3, 2, nullptr));
+  TRY(rhs_enqueue(s, false, 4, 3, nullptr));
+  TRY(rhs_enqueue(s, true, 1, 4, s->recs));
   return NSL_OK;
 }
 
@@ -511,7 +545,7 @@ static int step_enqueue(nsl_solver *s) {
 static int rhs_current(nsl_solver *s) {
   hipLaunchKernelGGL(k_set_active, dim3(1), dim3(1), 0, s->st, s->ctl, 1, (size_t)0);
   LAUNCH_CHECK();
-  TRY(rhs_enqueue(s, s->S, 1, 4, nullptr));
+  TRY(rhs_enqueue(s, true, 1, 4, nullptr));
   TRY(pull_ctl(s));
   diag_from(s->hctl->diag[4], s->last);
   return NSL_OK;
@@ -539,6 +573,71 @@ static int norms_dev(nsl_solver *s, const double2 *U, double *out3_dev) {
   LAUNCH_CHECK();
   return NSL_OK;
 }
+
+// ------------------------------------------------------------------------------------------------ FP32 solves
+// Widen the complex64 state to its FP64 view, or round the FP64 view into the complex64 state (n complex values).
+__global__ __launch_bounds__(NSL_NT) void k_widen(const float2 *__restrict__ a, double2 *__restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * NSL_NT + threadIdx.x; i < n; i += (long long)gridDim.x * NSL_NT)
+    b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+__global__ __launch_bounds__(NSL_NT) void k_narrow(const double2 *__restrict__ a, float2 *__restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * NSL_NT + threadIdx.x; i < n; i += (long long)gridDim.x * NSL_NT)
+    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// The FP64 buffers of an FP32 solve's host views, allocated for the view and freed after it: S (the state,
+// widened when `widen`), T and NL (scratch of the initial fields), X1 and X2 (transform scratch of the initial
+// fields and the diagnostics). A no-op for FP64 solves, whose S..X2 are their own.
+static int aux_acquire(nsl_solver *s, bool widen) {
+  if (!s->fp32) return NSL_OK;
+  const Geo &g = s->x.g;
+  if (!s->aux) {
+    size_t off = 0;
+    auto take = [&](size_t b) {
+      const size_t o = off;
+      off += (b + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t D3 = (size_t)3 * g.D * sizeof(double2);
+    const size_t oS = take(D3), oT = take(D3), oNL = take(D3);
+    const size_t oX1 = take((size_t)6 * g.N * g.by * g.hp * sizeof(double2));
+    const size_t oX2 = take((size_t)6 * g.N * g.N * g.hp * sizeof(double2));
+    char *p = nullptr;
+    const hipError_t e = hipMalloc((void **)&p, off);
+    if (e != hipSuccess)
+      return fail(NSL_ERR_HIP, "FP64 view of an FP32 solve: hipMalloc of " + std::to_string(off) + " bytes: " + hipGetErrorString(e));
+    s->aux = p;
+    s->S = (double2 *)(p + oS); s->T = (double2 *)(p + oT); s->NL = (double2 *)(p + oNL);
+    s->X1 = (double2 *)(p + oX1); s->X2 = (double2 *)(p + oX2);
+    HIP_TRY(hipMemsetAsync(p, 0, off, s->st));   // padding slots of S read as zeros
+  }
+  if (widen) {
+    hipLaunchKernelGGL(k_widen, dim3(sweep_grid(3 * g.D)), dim3(NSL_NT), 0, s->st, (const float2 *)s->S32, s->S, 3 * g.D);
+    LAUNCH_CHECK();
+  }
+  return NSL_OK;
+}
+// Round the FP64 view of the state into the FP32 state.
+static int aux_narrow(nsl_solver *s) {
+  const Geo &g = s->x.g;
+  hipLaunchKernelGGL(k_narrow, dim3(sweep_grid(3 * g.D)), dim3(NSL_NT), 0, s->st, (const double2 *)s->S, s->S32, 3 * g.D);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+static void aux_release(nsl_solver *s) {
+  if (!s->fp32 || !s->aux) return;
+  (void)hipStreamSynchronize(s->st);
+  (void)hipFree(s->aux);
+  s->aux = nullptr;
+  s->S = s->T = s->NL = s->X1 = s->X2 = nullptr;
+}
+// The FP64 view for the duration of one host call.
+struct AuxView {
+  nsl_solver *s;
+  int rc;
+  AuxView(nsl_solver *s_, bool widen) : s(s_), rc(aux_acquire(s_, widen)) {}
+  ~AuxView() { aux_release(s); }
+};
 
 // Mulberry32 (nslab.js:161): a = a + 0x6D2B79F5, then a 32-bit hash of a.
 static inline double mb32_next(uint32_t &a) {
@@ -613,6 +712,8 @@ static int set_ic(nsl_solver *s, int ic, const double *icp) {
   for (int q = 0; q < 4; q++) s->icp[q] = icp ? icp[q] : 0;
   const Geo &g = s->x.g;
   const int N = g.N;
+  AuxView av(s, false);   // FP32 solves: the field is built in FP64, then rounded into the FP32 state
+  TRY(av.rc);
   if (ic == NSL_IC_RANDOM) {
     TRY(ic_random(s, s->S, s->icp[0], s->icp[1], s->icp[2]));
   } else {
@@ -637,6 +738,10 @@ static int set_ic(nsl_solver *s, int ic, const double *icp) {
       LAUNCH_CHECK();
     }
   }
+  if (s->fp32) {
+    TRY(aux_narrow(s));
+    aux_release(s);
+  }
   // finishIC (nslab.js:293-299): counters reset, one rhs, E0/Z0 from it
   s->t = 0; s->step = 0; s->dt = 0; s->divMax = 0; s->diag_valid = 0;
   TRY(pull_ctl(s));
@@ -657,6 +762,7 @@ static void solver_free(nsl_solver *s) {
   if (s->dpart) (void)hipFree(s->dpart);
   if (s->hrecs) (void)hipHostFree(s->hrecs);
   if (s->hctl) (void)hipHostFree(s->hctl);
+  if (s->aux) (void)hipFree(s->aux);
   for (auto e : s->evs) (void)hipEventDestroy(e);
   if (s->evfence) (void)hipEventDestroy(s->evfence);
   if (!s->view) {
@@ -678,7 +784,8 @@ static int solver_host_init(nsl_solver *s) {
   return push_ctl(s);
 }
 
-static int create_impl(const nsl_opts *o, nsl_solver **out) {
+// fp32: complex64 storage of every stepped buffer (nslab_gpu.py:56-65), for the sizes with an FP32 path.
+static int create_impl(const nsl_opts *o, nsl_solver **out, bool fp32 = false) {
   if (!o || !out) return fail(NSL_ERR_ARG, "null argument");
   *out = nullptr;
   const int N = o->N;
@@ -686,15 +793,18 @@ static int create_impl(const nsl_opts *o, nsl_solver **out) {
   if (factorize_host(N, f) < 0) return fail(NSL_ERR_ARG, bad_n_msg(N));
   if (N % 2) return fail(NSL_ERR_ARG, "grid size must be even for the real-to-complex transforms (got " + std::to_string(N) + ")");
   if (o->ic < 0 || o->ic > NSL_IC_TUBESZ) return fail(NSL_ERR_ARG, "unknown initial condition " + std::to_string(o->ic));
+  if (fp32 && !fp32_has(N))
+    return fail(NSL_ERR_ARG, "FP32 storage is compiled for N = 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024 (got " + std::to_string(N) + ")");
   nsl_solver *s = new nsl_solver();
-  s->N = N; s->nu = o->nu; s->cfl = o->cfl; s->dtFixed = o->dt;
+  s->N = N; s->nu = o->nu; s->cfl = o->cfl; s->dtFixed = o->dt; s->fp32 = fp32 ? 1 : 0;
   if (hipGetDevice(&s->dev) != hipSuccess) { delete s; return fail(NSL_ERR_HIP, "hipGetDevice failed"); }
   hipError_t e = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete s; return fail(NSL_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
   int r = xform_init(s->x, geo_truncated(N), s->st);
   if (r != NSL_OK) { solver_free(s); return r; }
   s->nblk = nblk_of(s->x.g);
-  const SlotLayout L = slot_layout(s->x.g, s->nblk);
+  if (fp32 && !s->x.fast) { solver_free(s); return fail(NSL_ERR_ARG, "FP32 storage needs the size-specialised kernels"); }
+  const SlotLayout L = fp32 ? slot_layout(s->x.g, s->nblk, sizeof(float2), false) : slot_layout(s->x.g, s->nblk);
   char *base = nullptr;
   if ((r = dalloc(s, &base, L.bytes)) != NSL_OK) { solver_free(s); return r; }
   bind_slot(s, base, L);
@@ -1153,6 +1263,7 @@ static int set_state_impl(nsl_solver *s, const double *const planes[6]) {
     }
   }
   HIP_TRY(hipMemcpyAsync(s->S, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s->st));
+  if (s->fp32) TRY(aux_narrow(s));
   s->diag_valid = 0;
   return rhs_current(s);
 }
@@ -1295,6 +1406,7 @@ static int noalloc_scratch(int dev, double **ptr) {
 
 // nsl_spectrum with its partials in the device scratch: the same kernels and launch shape, so the same bits.
 static int spectrum_noalloc_impl(nsl_solver *s, double *Ek) {
+  if (s->fp32) return fail(NSL_ERR_STATE, "the allocation-free spectrum serves FP64 solves (an FP32 solve's spectrum reads an FP64 view it allocates)");
   const size_t need = spectrum_scratch_doubles(s) * sizeof(double);
   if (need > NSL_NOALLOC_SCRATCH_BYTES)
     return fail(NSL_ERR_ARG, "spectrum of N = " + std::to_string(s->N) + " needs " + std::to_string(need) +
@@ -1319,6 +1431,11 @@ int nsl_factorize(int32_t N, int32_t *f, int32_t cap) {
 }
 
 int nsl_create(const nsl_opts *opts, nsl_solver **out) { return create_impl(opts, out); }
+int nsl_create_fp32(const nsl_opts *opts, nsl_solver **out) { return create_impl(opts, out, true); }
+int nsl_precision(const nsl_solver *s) {
+  if (!s) return fail(NSL_ERR_ARG, "null solver");
+  return s->fp32 ? 32 : 64;
+}
 
 int nsl_set_ic(nsl_solver *s, int32_t ic, const double icp[4]) {
   if (!s) return fail(NSL_ERR_ARG, "null solver");
@@ -1360,6 +1477,8 @@ int nsl_last(nsl_solver *s, nsl_rhsdiag *out) {
 int nsl_norms(nsl_solver *s, double out[3]) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
   double *dn = (double *)((char *)s->ctl + offsetof(Ctl, nrm));
+  AuxView av(s, true);
+  TRY(av.rc);
   TRY(norms_dev(s, s->S, dn));
   HIP_TRY(hipMemcpyAsync(out, dn, 3 * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIP_TRY(hipStreamSynchronize(s->st));
@@ -1369,6 +1488,8 @@ int nsl_norms(nsl_solver *s, double out[3]) {
 int nsl_get_state(nsl_solver *s, double *const planes[6]) {
   if (!s || !planes) return fail(NSL_ERR_ARG, "null argument");
   for (int i = 0; i < 6; i++) if (!planes[i]) return fail(NSL_ERR_ARG, "null plane");
+  AuxView av(s, true);
+  TRY(av.rc);
   return get_state_impl(s, planes);
 }
 
@@ -1376,6 +1497,8 @@ int nsl_set_state(nsl_solver *s, const double *const planes[6]) {
   if (!s || !planes) return fail(NSL_ERR_ARG, "null argument");
   for (int i = 0; i < 6; i++) if (!planes[i]) return fail(NSL_ERR_ARG, "null plane");
   TRY(slot_in_flight(s));
+  AuxView av(s, false);
+  TRY(av.rc);
   return set_state_impl(s, planes);
 }
 
@@ -1415,6 +1538,8 @@ int nsl_spectral_inverse(int32_t N, const double *re, const double *im, double *
 
 int nsl_diagnose(nsl_solver *s, int32_t do_interp, double *out) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return diagnose_impl(s, do_interp, out);
 }
 
@@ -1436,6 +1561,8 @@ int nsl_spectrum_len(nsl_solver *s) {
 
 int nsl_spectrum(nsl_solver *s, double *Ek) {
   if (!s || !Ek) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return spectrum_impl(s, Ek);
 }
 
@@ -1446,33 +1573,50 @@ int nsl_spectrum_noalloc(nsl_solver *s, double *Ek) {
 
 int nsl_interp_max(nsl_solver *s, int32_t top, int32_t iters, double out[5]) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return interp_max_impl(s, top, iters, out);
 }
 
 int nsl_field(nsl_solver *s, int32_t which, double *out) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return field_impl(s, which, out);
 }
 
 int nsl_exact_error(nsl_solver *s, double out[2]) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return exact_error_impl(s, out);
 }
 
 int nsl_slice(nsl_solver *s, int32_t kind, int32_t axis, int32_t index, float *out) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return slice_impl(s, kind, axis, index, out);
 }
 
 int nsl_image_diag(nsl_solver *s, double out[4]) {
   if (!s || !out) return fail(NSL_ERR_ARG, "null argument");
+  AuxView av(s, true);
+  TRY(av.rc);
   return image_diag_impl(s, out);
 }
 
 int nsl_cutoff_pile_up(int32_t kc, const double *spec, int32_t nb, double *out) {
   if (!spec || !out) return fail(NSL_ERR_ARG, "null argument");
   if (kc < 1 || nb <= kc) return fail(NSL_ERR_ARG, "spectrum of " + std::to_string(nb) + " bins does not reach kc = " + std::to_string(kc));
-  return pile_up(kc, spec, nb, out);
+  return pile_up(kc, spec, nb, kEps64, out);
+}
+
+int nsl_cutoff_pile_up_eps(int32_t kc, const double *spec, int32_t nb, double eps, double *out) {
+  if (!spec || !out) return fail(NSL_ERR_ARG, "null argument");
+  if (kc < 1 || nb <= kc) return fail(NSL_ERR_ARG, "spectrum of " + std::to_string(nb) + " bins does not reach kc = " + std::to_string(kc));
+  if (!(eps > 0 && eps < 1)) return fail(NSL_ERR_ARG, "machine epsilon must lie in (0, 1)");
+  return pile_up(kc, spec, nb, eps, out);
 }
 
 int nsl_grid_size(const nsl_solver *s) {

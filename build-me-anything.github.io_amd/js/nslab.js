@@ -75,10 +75,15 @@ function createSolver(opts, pool) {
   const kc = Math.floor(N / 3);
   const ic0 = opts.ic || 'tgv';
   const v0 = icVector(ic0, opts.icParams);
+  // the GPU runner's fp32 flag (nslab_gpu.py:56-65): complex64 storage; machine epsilon of the working precision and
+  // the spectral noise floor it implies (E is quadratic in the field)
+  const fp32 = !!opts.fp32;
+  if (fp32 && pool) throw new Error('FP32 solves are never pool slots');
+  const eps = fp32 ? 1.1920929e-07 : 2.220446049250313e-16, specFloor = 100 * eps * eps;
   // a pool slot (Pool.admit) or a solve of its own
   let h, slot = -1;
   if (pool) [slot, h] = addon.poolAdmit(pool.h, N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3]);
-  else h = addon.create(N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3]);
+  else h = addon.create(N, IC_KIND[ic0], nu, cfl, dtFixed, v0[0], v0[1], v0[2], v0[3], fp32 ? 1 : 0);
   const st = { t: 0, step: 0, dt: 0, N, Re, nu, kc, ic: ic0, icParams: opts.icParams || {}, version: VERSION,
     series: { t: [], E: [], Z: [], eps: [], omMax: [], uMax: [], dt: [], ebal: [], zbal: [], Pspec: [], Pal: [] },
     maxEbal: 0, maxZbal: 0, maxTnl: 0, divMax: 0, outputs: [] };
@@ -159,8 +164,9 @@ function createSolver(opts, pool) {
   function norms() { sync(); const a = addon.norms(h); return { E: a[0], Z: a[1], P: a[2] }; }
   function spectrum() { sync(); return Array.from(addon.spectrum(h)); }
 
-  /** cutoffPileUp (nslab.js:557-564): the library's rule over a spectrum of this grid; null below the noise floor. */
-  function cutoffPileUp(spec) { return addon.cutoffPileUp(kc, Float64Array.from(spec)); }
+  /** cutoffPileUp (nslab.js:557-564): the library's rule over a spectrum of this grid; null below the noise floor of
+   *  the working precision (nslab_gpu.py:369-372 for an FP32 solve). */
+  function cutoffPileUp(spec) { return addon.cutoffPileUp(kc, Float64Array.from(spec), fp32 ? eps : undefined); }
 
   /** diagnose (nslab.js:339-392): the whole computation runs on the GPU; the result object has the reference's keys. */
   function diagnose() {
@@ -210,19 +216,20 @@ function createSolver(opts, pool) {
   }
 
   // what report.js reads of this solve
-  const run_ = { N, kc, nu, Re, cfl, dtFixed, st, get last() { return last; }, spectrum, pileUp: cutoffPileUp };
+  const run_ = { N, kc, nu, Re, cfl, dtFixed, st, fp32, get last() { return last; }, spectrum, pileUp: cutoffPileUp };
   /** Health report (nslab.js:595-635). */
   function health(study) { return report.healthReport(run_, study); }
   /** Evidence dossier (nslab.js:637-645). */
   function dossier(extra) {
-    return report.dossierOf(run_, 'Pocket Wind Tunnel · NSLab ' + VERSION + ' (MI355X FP64, ' + addon.version() + ')', extra);
+    return report.dossierOf(run_, 'Pocket Wind Tunnel · NSLab ' + VERSION + ' (MI355X ' + (fp32 ? 'FP32 storage' : 'FP64') + ', ' + addon.version() + ')', extra);
   }
   /** Frees the device memory now (a pool member is left to its pool); later calls on this solver throw. */
   function destroy() { if (!pool) addon.destroy(h); }
 
   finishIC();
   const solver = { st, get last() { return last; }, step, run, diagnose, spectrum, slice, exactError, health, dossier, setIC,
-    norms, interpMax, imageDiag, fields, N, nu, Re, kc, _sp: sp, destroy,
+    norms, interpMax, imageDiag, fields, N, nu, Re, kc, _sp: sp, destroy, fp32, eps, specFloor,
+    precision: fp32 ? 'float32' : 'float64',
     deviceBytes: () => addon.deviceBytes(h),
     profile: on => addon.profile(h, on ? 1 : 0), profileRead: () => addon.profileRead(h),
     _slot: slot, _book: book, _sync: sync };

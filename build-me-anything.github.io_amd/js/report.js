@@ -17,6 +17,14 @@ const DASH = '—';
 const NOTE = 'A run that does not PASS resolution, budgets and refinement cannot support any claim about the ' +
   'equations. Numerical growth ≠ mathematical singularity. The verdict includes the worst archived snapshot; ' +
   'worstEnd is the end-of-run verdict alone.';
+// FP32 solves (the GPU runner's grading, nslab_gpu.py:410-459): the rows that measure quantities exactly zero in
+// exact arithmetic (divergence after projection, net nonlinear transfer) are roundoff-limited in float32 and
+// reported ungraded; the note says what a float32 PASS does not cover.
+const ROUNDOFF = ' — float32: roundoff-limited, ungraded';
+const NOTE_FP32 = ' float32 run: the divergence and nonlinear-transfer rows are roundoff-limited and reported N/A ' +
+  'rather than graded, and the cutoff pile-up is ungraded whenever E(0.8·kc) sits on the fp32 noise floor. A PASS ' +
+  'here is therefore weaker than a float64 PASS — it is silent about those quantities, not clear on them. float32 ' +
+  'remains exploration grade (the expl- prefix); archived evidence is float64.';
 const EQUATIONS = '3D incompressible Navier–Stokes, periodic [0,2π]³, Re = 1/ν, Fourier pseudo-spectral ' +
   '(2/3 dealiased, rotational form), RK4';
 const DISCLAIMER = 'Numerical evidence only. Nothing here is a proof of global regularity or of finite-time blow-up ' +
@@ -52,13 +60,13 @@ const tailRatio = (spec, kc) => spec[kc] / Math.max(spec[peakBin(spec)], 1e-300)
 const stretchGap = (q, nu) => Math.abs(q.Pspec - q.Pphys) / Math.max(Math.abs(q.Pspec), 2 * nu * q.P, 1e-300);
 
 // The end-of-run checks, in the reference's row order. ctx: {N, kc, nu, dtFixed, st, d (last), o (last snapshot or
-// null), spec, pile, study}.
+// null), spec, pile, study, fp32}. `roundoff`: ungraded ('N/A', label suffixed) in an FP32 solve.
 const END_CHECKS = [
   { label: 'grid', text: c => `${c.N}³ (kmax ${c.kc}, dealiased 2/3)` },
   { label: 'Δt', text: c => c.st.dt.toExponential(3) + (c.dtFixed ? ' fixed' : ' adaptive') },
   { label: 'CFL (u_max Δt/Δx)', value: c => c.d.uMax * c.st.dt / (TWO_PI / c.N), format: fixed(3), limits: [0.8, 1.2] },
-  { label: 'divergence L∞', value: c => c.st.divMax, format: expo(2), limits: [1e-10, 1e-6] },
-  { label: 'nonlinear energy transfer |T|/ε', value: c => c.st.maxTnl, format: expo(2), limits: [1e-9, 1e-6] },
+  { label: 'divergence L∞', value: c => c.st.divMax, format: expo(2), limits: [1e-10, 1e-6], roundoff: true },
+  { label: 'nonlinear energy transfer |T|/ε', value: c => c.st.maxTnl, format: expo(2), limits: [1e-9, 1e-6], roundoff: true },
   { label: 'energy budget residual (RK4-consistent)', value: c => c.st.maxEbal, format: expo(2), limits: [1e-5, 1e-3] },
   { label: 'enstrophy budget residual', value: c => c.st.maxZbal, format: expo(2), limits: [1e-4, 1e-2] },
   { label: 'resolution kmax·η', value: c => c.kmaxEta, format: fixed(2), limits: [1.0, 0.5], higher: true },
@@ -91,6 +99,7 @@ function endRow(chk, c) {
   }
   if (chk.text) return [chk.label, chk.text(c), ''];
   const v = chk.value(c);
+  if (chk.roundoff && c.fp32) return [chk.label + ROUNDOFF, chk.format(v), 'N/A'];
   if (v == null) return [chk.label, DASH, ''];
   return [chk.label, chk.format(v), chk.limits ? grade(v, chk.limits, chk.higher) : ''];
 }
@@ -107,15 +116,17 @@ function instantRow(chk, snaps, c) {
 }
 
 /**
- * health(study) of a solve (nslab.js:595-635). run: {N, kc, nu, dtFixed, st, last, spectrum, pileUp}; `spectrum` is
- * the last snapshot's or, without one, the current state's; `pileUp(spec)` the cutoff pile-up rule.
+ * health(study) of a solve (nslab.js:595-635). run: {N, kc, nu, dtFixed, st, last, spectrum, pileUp, fp32}; `spectrum`
+ * is the last snapshot's or, without one, the current state's; `pileUp(spec)` the cutoff pile-up rule. An FP32 solve
+ * (fp32 true) is graded as the GPU runner grades one (nslab_gpu.py:410-459) and its report carries
+ * precision 'float32'.
  */
 function healthReport(run, study) {
   const { N, kc, nu, st } = run;
   const d = run.last, o = st.outputs.length ? st.outputs[st.outputs.length - 1] : null;
   const spec = o ? o.spectrum : run.spectrum();
   const eta = Math.pow(nu * nu * nu / Math.max(d.eps, 1e-300), 0.25);
-  const ctx = { N, kc, nu, dtFixed: run.dtFixed, st, d, o, spec, pile: run.pileUp(spec), study,
+  const ctx = { N, kc, nu, dtFixed: run.dtFixed, st, d, o, spec, pile: run.pileUp(spec), study, fp32: !!run.fp32,
     kmaxEta: Math.sqrt(3) * kc * eta };
   const rows = END_CHECKS.map(chk => endRow(chk, ctx));
   const worstEnd = verdictOf(rows);
@@ -125,6 +136,7 @@ function healthReport(run, study) {
       if (r) rows.push(r);
     }
   }
+  if (ctx.fp32) return { rows, worst: verdictOf(rows), worstEnd, precision: 'float32', note: NOTE + NOTE_FP32 };
   return { rows, worst: verdictOf(rows), worstEnd, note: NOTE };
 }
 
@@ -171,4 +183,4 @@ function studySummary(levels, key) {
   return { rows, verdict, text, order };
 }
 
-module.exports = { healthReport, dossierOf, studySummary, grade, verdictOf, NOTE, EQUATIONS, DISCLAIMER };
+module.exports = { healthReport, dossierOf, studySummary, grade, verdictOf, NOTE, NOTE_FP32, ROUNDOFF, EQUATIONS, DISCLAIMER };

@@ -6,7 +6,7 @@
  *
  *   node run_long.js --N 256 --Re 1600 --tEnd 10 --out <dir> [--ic tgv] [--cfl 0.4] [--dt 0] [--snap 0.5]
  *                    [--ckpt 2] [--chunk 16] [--icParams '{"seed":3}'] [--no-peak-track] [--bench]
- *                    [--compare <archived final.json>]
+ *                    [--compare <archived final.json>] [--fp32]
  *
  * Writes into <dir>: run.log, partial.json (rewritten at every snapshot), slices/t<t>_<kind>_<axis><i>.f32,
  * checkpoint.bin (the six state planes, float64, 6 x N*N*(N/2+1)) + checkpoint.json (atomic rename; a rerun with the
@@ -27,7 +27,7 @@ function parseArgs(argv) {
     N: +arg('N', 64), Re: +arg('Re', 1600), tEnd: +arg('tEnd', 10), ic: arg('ic', 'tgv'), cfl: +arg('cfl', 0.4),
     dt: +arg('dt', 0), snap: +arg('snap', 0.5), ckpt: +arg('ckpt', 2), chunk: +arg('chunk', 16),
     icParams: JSON.parse(arg('icParams', '{}')), peakTrack: !argv.includes('--no-peak-track'),
-    bench: argv.includes('--bench'), compare: arg('compare', null),
+    bench: argv.includes('--bench'), compare: arg('compare', null), fp32: argv.includes('--fp32'),
   };
   o.out = arg('out', path.join(process.cwd(), `${o.ic}-Re${o.Re}-N${o.N}`));
   return o;
@@ -42,9 +42,11 @@ function runLong(a) {
   };
   const build = { version: NS.VERSION, library: NS.addonVersion };
   const device = 'MI355X (gfx950), ' + NS.addonVersion;
-  log(`NSLab ${NS.VERSION} (${NS.addonVersion}) — ${a.ic} N=${a.N}³ Re=${a.Re} tEnd=${a.tEnd} cfl=${a.cfl} snap=${a.snap} ckpt=${a.ckpt}  node ${process.version}`);
+  // the header ends with the working precision, as the GPU runner's does (nslab_gpu.py:485)
+  const precision = a.fp32 ? 'float32' : 'float64';
+  log(`NSLab ${NS.VERSION} (${NS.addonVersion}) — ${a.ic} N=${a.N}³ Re=${a.Re} tEnd=${a.tEnd} cfl=${a.cfl} snap=${a.snap} ckpt=${a.ckpt}  node ${process.version}  ${precision}`);
   const t0 = Date.now();
-  const s = NS.createSolver({ N: a.N, Re: a.Re, ic: a.ic, cfl: a.cfl, dt: a.dt, icParams: a.icParams });
+  const s = NS.createSolver({ N: a.N, Re: a.Re, ic: a.ic, cfl: a.cfl, dt: a.dt, icParams: a.icParams, fp32: a.fp32 });
   log(`solver ready in ${((Date.now() - t0) / 1000).toFixed(1)} s, device ${(s.deviceBytes() / 1e9).toFixed(2)} GB, kc ${s.kc}, E0 ${s.st.E0}, Z0 ${s.st.Z0}`);
   // run until the first step with t >= tStop (the reference checks `t >= next - 1e-9` after every step)
   const runTo = (tStop, maxSteps) => {
@@ -107,9 +109,14 @@ function runLong(a) {
       log(`resumed from checkpoint at t=${meta.t.toFixed(4)} step ${meta.step} (${outs.length} snapshots)`);
     } else log('checkpoint present but for a different case — ignored');
   }
-  const health = () => { const h = s.health(); return { worst: h.worst, worstEnd: h.worstEnd, rows: h.rows, note: h.note }; };
+  const health = () => {
+    const h = s.health();
+    const o = { worst: h.worst, worstEnd: h.worstEnd, rows: h.rows, note: h.note };
+    if (h.precision) o.precision = h.precision;
+    return o;
+  };
   function writePartial(final) {
-    const j = { instrument: 'Pocket Wind Tunnel NSLab ' + NS.VERSION + ' (MI355X)', build, device, precision: 'float64',
+    const j = { instrument: 'Pocket Wind Tunnel NSLab ' + NS.VERSION + ' (MI355X)', build, device, precision,
       case: { ic: a.ic, N: a.N, Re: a.Re, nu: 1 / a.Re, cfl: a.cfl, tEnd: a.tEnd, snapEvery: a.snap, icParams: a.icParams },
       t: s.st.t, steps: s.st.step, elapsed_s: elapsedBefore + (Date.now() - t0) / 1000, series: s.st.series,
       snapshots: outs, peakTrack, health: health(), final: !!final,

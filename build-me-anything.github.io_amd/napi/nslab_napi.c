@@ -98,21 +98,23 @@ static nsl_solver *get_solver(napi_env env, napi_value v) {
   if (!s) { napi_throw_type_error(env, NULL, "not a solver handle"); return NULL; }
 
 /* create(N, ic, nu, cfl, dt, icp0..icp3) -> handle */
+/* create(N, ic, nu, cfl, dt, p0, p1, p2, p3[, fp32]): fp32 truthy selects complex64 storage (nsl_create_fp32) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
-  ARGS(9);
+  ARGS(10);
   nsl_opts o;
   memset(&o, 0, sizeof o);
-  int64_t N = 0, ic = 0;
+  int64_t N = 0, ic = 0, fp32 = 0;
   if (argc < 9 || !get_i64(env, argv[0], &N) || !get_i64(env, argv[1], &ic) || !get_f64(env, argv[2], &o.nu) ||
       !get_f64(env, argv[3], &o.cfl) || !get_f64(env, argv[4], &o.dt)) {
-    napi_throw_type_error(env, NULL, "create(N, ic, nu, cfl, dt, p0, p1, p2, p3)");
+    napi_throw_type_error(env, NULL, "create(N, ic, nu, cfl, dt, p0, p1, p2, p3[, fp32])");
     return NULL;
   }
   for (int i = 0; i < 4; i++) if (!get_f64(env, argv[5 + i], &o.icp[i])) o.icp[i] = 0;
+  if (argc >= 10 && !get_i64(env, argv[9], &fp32)) fp32 = 0;
   o.N = (int32_t)N;
   o.ic = (int32_t)ic;
   nsl_solver *s = NULL;
-  int rc = nsl_create(&o, &s);
+  int rc = fp32 ? nsl_create_fp32(&o, &s) : nsl_create(&o, &s);
   if (rc < 0) return throw_status(env, rc);
   return wrap_solver(env, s, 1);
 }
@@ -351,22 +353,36 @@ static napi_value js_image_diag(napi_env env, napi_callback_info info) {
 }
 
 /* cutoffPileUp(kc, Float64Array spectrum) -> number, or null below the noise floor */
+/* cutoffPileUp(kc, spectrum[, eps]): eps = machine epsilon of the working precision (FP64 when absent) */
 static napi_value js_cutoff_pile_up(napi_env env, napi_callback_info info) {
-  ARGS(2);
+  ARGS(3);
   int64_t kc = 0;
   double *sp;
   size_t nb;
   if (!get_i64(env, argv[0], &kc) || !get_f64arr(env, argv[1], &sp, &nb)) {
-    napi_throw_type_error(env, NULL, "cutoffPileUp(kc, Float64Array spectrum)");
+    napi_throw_type_error(env, NULL, "cutoffPileUp(kc, Float64Array spectrum[, eps])");
     return NULL;
   }
-  double v = 0;
-  int rc = nsl_cutoff_pile_up((int32_t)kc, sp, (int32_t)nb, &v);
+  double v = 0, eps = 0;
+  const int has_eps = argc >= 3 && get_f64(env, argv[2], &eps);
+  int rc = has_eps ? nsl_cutoff_pile_up_eps((int32_t)kc, sp, (int32_t)nb, eps, &v)
+                   : nsl_cutoff_pile_up((int32_t)kc, sp, (int32_t)nb, &v);
   if (rc < 0) return throw_status(env, rc);
   napi_value out;
   if (rc == 0) CALL(env, napi_get_null(env, &out));
   else CALL(env, napi_create_double(env, v, &out));
   return out;
+}
+
+/* precision(h): 64 or 32 */
+static napi_value js_precision(napi_env env, napi_callback_info info) {
+  ARGS(1);
+  SOLVER(0);
+  const int rc = nsl_precision(s);
+  if (rc < 0) return throw_status(env, rc);
+  napi_value v;
+  CALL(env, napi_create_int32(env, rc, &v));
+  return v;
 }
 
 static napi_value js_device_bytes(napi_env env, napi_callback_info info) {
@@ -839,6 +855,7 @@ static napi_value Init(napi_env env, napi_value exports) {
     {"tuneGet", js_tune_get}, {"tuneSet", js_tune_set}, {"tuneCandidates", js_tune_candidates},
     {"slice", js_slice}, {"imageDiag", js_image_diag}, {"cutoffPileUp", js_cutoff_pile_up},
     {"groupCreate", js_group_create}, {"groupLaunch", js_group_launch}, {"groupDestroy", js_group_destroy},
+    {"precision", js_precision},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

@@ -69,6 +69,16 @@ typedef struct {
 /* createSolver(opts) (nslab.js:171): allocates the device state, builds the IC and runs the first rhs.
  * On success *out is a new solver; st.E0/st.Z0 are last.E/last.Z of the returned nsl_last(). */
 int nsl_create(const nsl_opts *opts, nsl_solver **out);
+/* Solver(N, Re, ic, icParams, cfl, dt, xp, fp32=True) of the GPU runner (nslab_gpu.py:56-65): the same solve with
+ * every stepped array (state, RK4 registers, each pass's output) held in complex64, half the bytes of nsl_create's
+ * FP64 solve; the arithmetic inside each pass stays FP64 in registers, so results carry FP32 rounding at every pass
+ * boundary. For N = 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024 (NSL_ERR_ARG otherwise). Never a pool slot. The
+ * host views (initial field, nsl_get_state/nsl_set_state, nsl_norms and the diagnostics) allocate an FP64 copy of
+ * the state and its transform scratch for the duration of the call, and nsl_spectrum_noalloc refuses an FP32 solve
+ * with NSL_ERR_STATE. */
+int nsl_create_fp32(const nsl_opts *opts, nsl_solver **out);
+/* 64 or 32: the storage precision of a solve (the runner's fp32 flag, nslab_gpu.py:59). */
+int nsl_precision(const nsl_solver *s);
 /* solver.setIC(kind, prm) (nslab.js:250): resets t and step to 0 and recomputes last. */
 int nsl_set_ic(nsl_solver *s, int32_t ic, const double icp[4]);
 void nsl_destroy(nsl_solver *s);
@@ -146,6 +156,9 @@ int nsl_image_diag(nsl_solver *s, double out[4]);
  * E(k)/E(k8) for k8 = round(0.8 kc) <= k <= kc. Returns 1 and *out, or 0 (the reference's null) when E(k8) is not
  * above 100 eps^2 times the largest E(k), k >= 1; NSL_ERR_ARG unless 1 <= kc < nb. Host arithmetic only. */
 int nsl_cutoff_pile_up(int32_t kc, const double *spec, int32_t nb, double *out);
+/* The same guard in a working precision of machine epsilon eps (the GPU runner's pileUp with specFloor = 100 eps^2,
+ * nslab_gpu.py:61-65,357-372): eps = 1.1920929e-07 for an FP32 solve. NSL_ERR_ARG unless 0 < eps < 1. */
+int nsl_cutoff_pile_up_eps(int32_t kc, const double *spec, int32_t nb, double eps, double *out);
 /* solver.N: the grid size of a solver or pool slot. */
 int nsl_grid_size(const nsl_solver *s);
 

@@ -588,13 +588,18 @@ void or_spectrum(or_solver *s, double *Ek) {
 }
 
 /* 1 when defined (result in *out), 0 = null. kc = floor(N/3) of the grid the spectrum belongs to. */
-int or_cutoff_pile_up(int kc, const double *spec, int nb, double *out) {
+/* The GPU runner's pileUp (nslab_gpu.py:369-372) in a working precision of machine epsilon eps: specFloor =
+ * 100 eps^2 (nslab_gpu.py:64-65); nslab.js:557-564 is the eps = 2^-52 case. */
+int or_cutoff_pile_up_eps(int kc, const double *spec, int nb, double eps, double *out) {
   const int k8 = (int)floor(0.8 * kc + 0.5); double peak = 0;
   for (int k = 1; k < nb; k++) peak = spec[k] > peak ? spec[k] : peak;
-  const double SPEC_FLOOR = 100 * 2.220446049250313e-16 * 2.220446049250313e-16;
+  const double SPEC_FLOOR = 100.0 * eps * eps;
   if (!(spec[k8] > SPEC_FLOOR * peak)) return 0;
   double m = 0; for (int k = k8; k <= kc; k++) { const double q = spec[k] / spec[k8]; m = q > m ? q : m; }
   *out = m; return 1;
+}
+int or_cutoff_pile_up(int kc, const double *spec, int nb, double *out) {
+  return or_cutoff_pile_up_eps(kc, spec, nb, 2.220446049250313e-16, out);
 }
 static int cutoffPileUp(or_solver *s, const double *spec, int nb, double *out) { return or_cutoff_pile_up(s->kc, spec, nb, out); }
 

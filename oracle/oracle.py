@@ -62,6 +62,8 @@ def lib():
         L.or_spectral_inverse.argtypes = [ctypes.c_int, dp, dp, dp]
         L.or_cutoff_pile_up.restype = ctypes.c_int
         L.or_cutoff_pile_up.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp]
+        L.or_cutoff_pile_up_eps.restype = ctypes.c_int
+        L.or_cutoff_pile_up_eps.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp]
         _lib = L
     return _lib
 
@@ -106,11 +108,15 @@ def fft1(xr, xi, sign):
     return re, im
 
 
-def cutoff_pile_up(kc, spec):
-    """cutoffPileUp (nslab.js:557-564) of a shell spectrum on a grid with kc = floor(N/3); None below the floor."""
+def cutoff_pile_up(kc, spec, eps=None):
+    """cutoffPileUp (nslab.js:557-564) of a shell spectrum on a grid with kc = floor(N/3); None below the floor.
+    eps: machine epsilon of the working precision (the GPU runner's specFloor = 100 eps^2, nslab_gpu.py:61-65);
+    None is FP64."""
     sp = np.ascontiguousarray(spec, dtype=np.float64)
     out = np.zeros(1)
-    return float(out[0]) if lib().or_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out)) else None
+    if eps is None:
+        return float(out[0]) if lib().or_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out)) else None
+    return float(out[0]) if lib().or_cutoff_pile_up_eps(int(kc), _p(sp), len(sp), float(eps), _p(out)) else None
 
 
 def spectral_forward(f):

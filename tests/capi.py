@@ -54,6 +54,8 @@ def lib():
         dp = ctypes.POINTER(ctypes.c_double)
         vp = ctypes.c_void_p
         L.nsl_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.nsl_create_fp32.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.nsl_precision.argtypes = [vp]
         L.nsl_set_ic.argtypes = [vp, ctypes.c_int32, dp]
         L.nsl_destroy.argtypes = [vp]
         L.nsl_destroy.restype = None
@@ -104,6 +106,7 @@ def lib():
         L.nsl_slice.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
         L.nsl_image_diag.argtypes = [vp, dp]
         L.nsl_cutoff_pile_up.argtypes = [ctypes.c_int32, dp, ctypes.c_int32, dp]
+        L.nsl_cutoff_pile_up_eps.argtypes = [ctypes.c_int32, dp, ctypes.c_int32, ctypes.c_double, dp]
         L.nsl_grid_size.argtypes = [vp]
         L.nsl_xpass_plan.argtypes = [ctypes.c_int32, ctypes.POINTER(Tune), ctypes.c_int32, ctypes.c_int64, i64p]
         L.nsl_last_error.restype = ctypes.c_char_p
@@ -161,11 +164,15 @@ def ic_params(kind, prm=None):
 class Solver:
     """createSolver(opts) through the C ABI, with the reference's st/series bookkeeping (nslab.js:188-190, 300-302)."""
 
-    def __init__(self, N=32, Re=None, nu=None, cfl=None, dt=None, ic="tgv", icParams=None, pool=None):
+    def __init__(self, N=32, Re=None, nu=None, cfl=None, dt=None, ic="tgv", icParams=None, pool=None, fp32=False):
         o = make_opts(N, Re, nu, cfl, dt, ic, icParams)
         h = ctypes.c_void_p()
         self.pool, self.slot = pool, None
-        if pool is None:
+        if fp32:   # the GPU runner's fp32 mode (nslab_gpu.py:56-65): complex64 storage
+            if pool is not None:
+                raise ValueError("FP32 solves are never pool slots")
+            check(lib().nsl_create_fp32(ctypes.byref(o), ctypes.byref(h)))
+        elif pool is None:
             check(lib().nsl_create(ctypes.byref(o), ctypes.byref(h)))
         else:   # a slot of a pool: createSolver into the slot, then the slot's solver handle
             sl = ctypes.c_int32(-1)
@@ -226,6 +233,10 @@ class Solver:
         return total
 
     @property
+    def precision(self):
+        return check(lib().nsl_precision(self.h))
+
+    @property
     def last(self):
         d = RhsDiag()
         check(lib().nsl_last(self.h, ctypes.byref(d)))
@@ -260,6 +271,12 @@ class Solver:
         nb = check(lib().nsl_spectrum_len(self.h))
         o = np.zeros(nb)
         check(lib().nsl_spectrum(self.h, _p(o)))
+        return o
+
+    def spectrum_noalloc(self):
+        nb = check(lib().nsl_spectrum_len(self.h))
+        o = np.zeros(nb)
+        check(lib().nsl_spectrum_noalloc(self.h, _p(o)))
         return o
 
     def diagnose(self, interp=True):
@@ -313,6 +330,14 @@ class Solver:
         return lib().nsl_device_bytes(self.h)
 
 
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f = ctypes.c_int64(0)
+    t = ctypes.c_int64(0)
+    check(lib().nsl_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
 def fft1(xr, xi, sign):
     re = np.ascontiguousarray(xr, dtype=np.float64).copy()
     im = np.ascontiguousarray(xi, dtype=np.float64).copy()
@@ -337,11 +362,16 @@ def spectral_inverse(N, re, im):
     return f
 
 
-def cutoff_pile_up(kc, spec):
-    """The library's cutoff pile-up of a shell spectrum (host arithmetic; no GPU needed). None below the floor."""
+def cutoff_pile_up(kc, spec, eps=None):
+    """The library's cutoff pile-up of a shell spectrum (host arithmetic; no GPU needed). None below the floor
+    100 eps^2 * peak; eps None: the FP64 guard of nsl_cutoff_pile_up, else nsl_cutoff_pile_up_eps."""
     sp = np.ascontiguousarray(spec, dtype=np.float64)
     out = np.zeros(1)
-    return float(out[0]) if check(lib().nsl_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out))) == 1 else None
+    if eps is None:
+        rc = check(lib().nsl_cutoff_pile_up(int(kc), _p(sp), len(sp), _p(out)))
+    else:
+        rc = check(lib().nsl_cutoff_pile_up_eps(int(kc), _p(sp), len(sp), float(eps), _p(out)))
+    return float(out[0]) if rc == 1 else None
 
 
 def xpass_plan(N, cfg, cus=256, cap_pairs=None):

@@ -69,5 +69,45 @@ for (const c of G('pileup_fp64.json').cases) {
   else same(`pile-up ${c.N}^3 t ${c.t.toFixed(2)}`, got, c.pileUp);
 }
 
+// FP32 solves are graded as the GPU runner grades its FP32 mode (nslab_gpu.py:410-459): the strings come from the
+// runner's source (tests/golden/formats.json, fp32_grading, extracted by make_format_fixture.py)
+const F = G('formats.json').fp32_grading;
+same('FP32 row-label suffix is the runner\'s', report.ROUNDOFF, F.label_suffix);
+same('FP32 note text is the runner\'s', report.NOTE_FP32, F.note_fp32);
+{
+  const r64 = runOf(32, 1600, 0.01, tgv.st200, tgv.diag, true), h64 = report.healthReport(r64, undefined);
+  const r32 = Object.assign(runOf(32, 1600, 0.01, tgv.st200, tgv.diag, true), { fp32: true });
+  // roundoff-level values an FP32 run produces: graded FAIL in FP64, ungraded in FP32
+  r32.st.divMax = 3.8e-6; r32.st.maxTnl = 5.7e-6;
+  const h32 = report.healthReport(r32, undefined);
+  const want = h64.rows.map(r => r.slice());
+  want[3] = ['divergence L∞' + F.label_suffix, '3.80e-6', 'N/A'];
+  want[4] = ['nonlinear energy transfer |T|/ε' + F.label_suffix, '5.70e-6', 'N/A'];
+  same('FP32 health rows: the two roundoff rows ungraded, every other row as in FP64', h32.rows, want);
+  same('FP32 verdicts ignore the ungraded rows', [h32.worst, h32.worstEnd], [h64.worst, h64.worstEnd]);
+  same('FP32 report: precision and note', [h32.precision, h32.note], [F.precision.fp32, report.NOTE + F.note_fp32]);
+  same('FP64 report carries neither', [h64.precision, h64.note], [undefined, report.NOTE]);
+  // the same roundoff values in an FP64 run fail
+  const r64b = runOf(32, 1600, 0.01, tgv.st200, tgv.diag, true);
+  r64b.st.divMax = 3.8e-6; r64b.st.maxTnl = 5.7e-6;
+  const hb = report.healthReport(r64b, undefined);
+  same('FP64 grades the same values FAIL', [hb.rows[3][2], hb.rows[4][2], hb.worstEnd], ['FAIL', 'FAIL', 'FAIL']);
+}
+// the host's FP32 pile-up guard (the library's, with the runner's FP32 epsilon) on the archived 640^3 FP32 run:
+// ungraded through t = 5.0, graded from t = 5.5 on (gpu/README.md:44-58)
+{
+  const d = G('pileup_fp32.json');
+  const graded = [];
+  for (const q of d.snapshots) {
+    const sp = new Float64Array(q.nb);
+    sp[1] = q.peak;
+    sp.set(q.band, d.k8);
+    const v = addon.cutoffPileUp(d.kc, sp, F.eps32);
+    if (v !== null) { graded.push(+q.t.toFixed(1)); if (v !== q.pileUp) same(`FP32 pile-up t ${q.t}`, v, q.pileUp); }
+    if (addon.cutoffPileUp(d.kc, sp) !== q.pileUp) same(`FP64 pile-up t ${q.t}`, addon.cutoffPileUp(d.kc, sp), q.pileUp);
+  }
+  same('FP32 guard on the archived 640^3 run: first graded snapshot, count', [graded[0], graded.length], [5.5, d.snapshots.length - 11]);
+}
+
 console.log(fails ? `${fails} FAILED` : 'ALL PASS');
 process.exit(fails ? 1 : 0);

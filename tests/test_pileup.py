@@ -140,3 +140,86 @@ def test_mutants_of_the_rule_are_caught():
     assert passes(lambda kc, sp: rule(kc, sp))
     for name, m in mutants.items():
         assert not passes(lambda kc, sp, m=m: rule(kc, sp, **m)), name
+
+
+# ------------------------------------------------------------------------------------------------ FP32 guard
+# The GPU runner's guard in its FP32 mode is 100 eps^2 with eps = 1.1920929e-07 (nslab_gpu.py:61-65,369-372).
+# tests/golden/pileup_fp32.json is the archived 640^3 FP32 run (runner 0.1.2, fixed 1e-20 guard), and the runner's
+# 0.1.3 notes (gpu/README.md:44-58) say what the precision-scaled guard does to it: t = 0.5 (E(k8)/peak 8.76e-20,
+# archived 15.38) is ungraded, every snapshot up to t = 5.0 is ungraded, and grading starts at t = 5.5, where
+# E(k8)/peak = 1.9e-12.
+EPS32 = 1.1920929e-07
+IMPLS_EPS = {"oracle": O.cutoff_pile_up, "library": capi.cutoff_pile_up}
+
+
+def fp32_run():
+    with open(os.path.join(HERE, "golden", "pileup_fp32.json")) as f:
+        d = json.load(f)
+    for s in d["snapshots"]:
+        sp = np.zeros(s["nb"])
+        sp[1] = s["peak"]
+        sp[d["k8"]:d["kc"] + 1] = s["band"]
+        s["spectrum"] = sp
+    return d
+
+
+@pytest.mark.parametrize("impl", sorted(IMPLS_EPS))
+def test_fp32_guard_on_the_archived_640_run(impl):
+    f = IMPLS_EPS[impl]
+    d = fp32_run()
+    kc = d["kc"]
+    assert d["N"] == 640 and kc == 213 and d["k8"] == 170
+    graded = []
+    for s in d["snapshots"]:
+        # the FP64 guard (4.9e-30 of the peak) grades every snapshot, as the old 1e-20 guard did: the archived values
+        assert f(kc, s["spectrum"]) == s["pileUp"], s["t"]
+        assert f(kc, s["spectrum"], eps=np.finfo(np.float64).eps) == s["pileUp"], s["t"]
+        v = f(kc, s["spectrum"], eps=EPS32)
+        if v is not None:
+            assert v == s["pileUp"], s["t"]
+            graded.append(round(s["t"], 1))
+    # the README's account: ungraded through t = 5.0 (15.38 at t = 0.5 among them), graded from t = 5.5 on
+    assert graded[0] == 5.5 and len(graded) == len(d["snapshots"]) - 11, graded
+    assert [round(s["t"], 1) for s in d["snapshots"]][11] == 5.5
+    t05 = d["snapshots"][1]
+    assert abs(t05["pileUp"] - 15.38) < 0.005 and f(kc, t05["spectrum"], eps=EPS32) is None
+
+
+@pytest.mark.parametrize("impl", sorted(IMPLS_EPS))
+def test_fp32_floor_closed_form(impl):
+    f = IMPLS_EPS[impl]
+    floor32 = 100 * EPS32 * EPS32
+    for kc in (16, 85, 213):
+        k8 = int(np.floor(0.8 * kc + 0.5))
+        band = np.linspace(2.0, 3.0, kc - k8)
+        assert f(kc, synthetic(kc, 1e3 * floor32 * 1.01, band, 1e3), eps=EPS32) is not None
+        assert f(kc, synthetic(kc, 1e3 * floor32 * 0.99, band, 1e3), eps=EPS32) is None
+        # between the two floors: graded in FP64, ungraded in FP32
+        sp = synthetic(kc, 1e3 * 1e-20, band, 1e3)
+        assert f(kc, sp) is not None and f(kc, sp, eps=EPS32) is None
+
+
+def test_library_refuses_a_bad_epsilon():
+    for eps in (0.0, -1e-7, 1.0, float("nan")):
+        with pytest.raises(capi.NslError, match="machine epsilon"):
+            capi.cutoff_pile_up(16, np.ones(40), eps=eps)
+
+
+def test_mutants_of_the_fp32_guard_are_caught():
+    """A floor of 100 eps (not squared), the FP64 epsilon in FP32 mode, or eps^2 without the margin of 100 each
+    disagree with the archived run's account above."""
+    d = fp32_run()
+    kc, k8 = d["kc"], d["k8"]
+
+    def graded_from(floor):
+        out = []
+        for s in d["snapshots"]:
+            sp = s["spectrum"]
+            if sp[k8] > floor * max(sp[1:]):
+                out.append(round(s["t"], 1))
+        return out
+
+    assert graded_from(100 * EPS32 ** 2)[0] == 5.5
+    for floor in (100 * EPS32, 100 * np.finfo(np.float64).eps ** 2, EPS32 ** 2):
+        g = graded_from(floor)
+        assert not (g and g[0] == 5.5 and len(g) == len(d["snapshots"]) - 11), floor
