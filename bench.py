@@ -9,14 +9,15 @@ event pair around every launch, for the per-kernel averages (the events add gaps
 
 Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
 its mean launch time, from HIP events recorded on the solver's stream inside the second timed region), the whole-step
-roofline fraction of the truncated-storage traffic model, a 512^3 line, the CPU baseline (the oracle, a
+roofline fraction of the truncated-storage traffic model, a 512^3 line, FP32-storage lines at the headline size
+and 512^3 (the GPU runner's fp32 mode: complex64 in HBM, FP64 arithmetic in registers), the CPU baseline (the oracle, a
 single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and three lines
 from the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step),
 the ladder request loop (Poisson arrivals, admission by HBM footprint, p50/p99 latency, ladders per hour, against
 sequential single-solve execution of the same work), and the same loop with half the requests Reynolds sweeps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--N 256] [--no-512] [--no-cpu] [--no-ensemble]
-                  [--no-serve] [--serve-requests R]
+                  [--no-serve] [--no-fp32] [--serve-requests R]
 """
 import argparse
 import ctypes
@@ -39,13 +40,12 @@ def geometry(N):
     return h, b, b * b * h
 
 
-def alg_bytes_per_launch(N, fused=False):
+def alg_bytes_per_launch(N, fused=False, c=16.0):
     """Algorithmic HBM bytes per launch of each kernel class: every operand read once and written once
-    (complex FP64 = 16 B). h = floor(N/3)+1 retained kx, b = 2 floor(N/3)+1 retained ky/kz, D = b*b*h.
-    fused: the z-forward runs inside the update (one kernel, timed as k_update<s>): its 3 pencils come in and the
-    nonlinear term never goes to HBM."""
+    (c bytes per stored complex value: 16 for FP64 solves, 8 for FP32 solves). h = floor(N/3)+1 retained kx,
+    b = 2 floor(N/3)+1 retained ky/kz, D = b*b*h. fused: the z-forward runs inside the update (one kernel, timed as
+    k_update<s>): its 3 pencils come in and the nonlinear term never goes to HBM."""
     h, b, D = geometry(N)
-    c = 16.0
     z = 3 * h * b * N   # z-forward input pencils
     return {
         "k_zinv": (3 * D + 6 * h * b * N) * c,           # 3 velocity fields in, u and omega (6) out
@@ -67,10 +67,12 @@ def b_alg_step(N):
     return 1152.0 * h * (b * b + b * N + N * N)
 
 
-def measure(N, steps, warmup, dt, label):
+def measure(N, steps, warmup, dt, label, fp32=False):
+    """fp32: an FP32 solve (complex64 storage, FP64 arithmetic in registers): 8 bytes per stored complex value."""
     import capi
     import torch
-    s = capi.Solver(N=N, Re=1600, ic="tgv", dt=dt)
+    s = capi.Solver(N=N, Re=1600, ic="tgv", dt=dt, fp32=fp32)
+    esz = 8.0 if fp32 else 16.0
     s.run(warmup)
     torch.cuda.synchronize()
     # timed region 1: the throughput, no instrumentation in the stream
@@ -94,7 +96,7 @@ def measure(N, steps, warmup, dt, label):
     assert done == steps and done_p == steps, (done, done_p, steps)
     elapsed = t1 - t0
     fused = cnt[CLASSES.index("k_zfwd")] == 0   # no separate z-forward launches: the fused pass ran
-    per = alg_bytes_per_launch(N, fused)
+    per = alg_bytes_per_launch(N, fused, esz)
     kern = {}
     for i, name in enumerate(CLASSES):
         if cnt[i]:
@@ -103,12 +105,13 @@ def measure(N, steps, warmup, dt, label):
                           "alg_bytes": per[name], "GBs": per[name] / avg_s / 1e9 if per[name] else 0.0}
     dom = max((k for k in kern if per[k] > 0), key=lambda k: kern[k]["total_ms"])
     E = s.last["E"]
+    dev_bytes = s.device_bytes()
     s.close()
     del s
     return {"label": label, "N": N, "steps": steps, "elapsed_s": elapsed, "steps_per_s": steps / elapsed,
             "ms_per_step": 1e3 * elapsed / steps, "ms_per_step_profiled": 1e3 * (tp1 - tp0) / steps,
-            "kernels": kern, "dominant": dom, "E_end": E, "zfwd_fused": fused,
-            "step_roofline_frac": b_alg_step(N) * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
+            "kernels": kern, "dominant": dom, "E_end": E, "zfwd_fused": fused, "device_bytes": dev_bytes,
+            "step_roofline_frac": b_alg_step(N) * esz / 16.0 * steps / elapsed / (HBM_PEAK_GBS * 1e9)}
 
 
 def measure_tubes(N, steps):
@@ -320,6 +323,7 @@ def main():
     ap.add_argument("--no-tubes", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=30)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one device-to-device copy of this many MiB first (PMC counter calibration)")
@@ -395,6 +399,23 @@ def main():
                        "dominant_GBs": extra["kernels"][extra["dominant"]]["GBs"],
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
                                    for n, v in extra["kernels"].items()}}
+    if not a.no_fp32:
+        # the GPU runner's FP32 mode: complex64 in HBM (half the bytes of every pass), FP64 arithmetic in registers
+        f32 = [measure(a.N, a.steps, a.warmup, a.dt, "fp32", fp32=True)]
+        if not a.no_512 and a.N != 512:
+            f32.append(measure(512, max(4, a.steps // 4), 1, 0.002, "fp32 512^3", fp32=True))
+        fp64_by_n = {a.N: main_r, 512: extra}
+        out["fp32"] = [{
+            "workload": "Taylor-Green %d^3, FP32 storage (complex64), FP64 arithmetic in registers, dt=%g fixed"
+                        % (r["N"], a.dt if r["N"] == a.N else 0.002),
+            "steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
+            "vs_fp64": r["steps_per_s"] / fp64_by_n[r["N"]]["steps_per_s"] if fp64_by_n.get(r["N"]) else None,
+            "step_roofline_frac": r["step_roofline_frac"], "b_alg_per_step_GB": b_alg_step(r["N"]) / 2e9,
+            "device_bytes": r["device_bytes"],
+            "device_bytes_vs_fp64": r["device_bytes"] / fp64_by_n[r["N"]]["device_bytes"] if fp64_by_n.get(r["N"]) else None,
+            "dominant": r["dominant"], "dominant_GBs": r["kernels"][r["dominant"]]["GBs"],
+            "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)} for n, v in r["kernels"].items()}}
+            for r in f32]
     if not a.no_tubes:
         # config 3: 500 RK4 steps at 512^3 (the full run), 3 at 1024^3
         out["tubes"] = [measure_tubes(512, 500), measure_tubes(1024, 3)]

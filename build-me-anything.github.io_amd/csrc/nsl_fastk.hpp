@@ -28,13 +28,19 @@ __host__ __device__ constexpr Geo trunc_geo() {
 template <int N> struct PassNT { static constexpr int v = N >= 512 ? NSL_PNT_LARGE : NSL_NT; };
 
 // Pencil passes store whole tiles, the padding columns kx = hx .. hp-1 included (zeros), so each 128-byte line is
-// written in full.
+// written in full. FP64 tiles (T lines of 16 bytes) divide the row pitch hp; the y passes' FP32 tiles of 16 lines
+// (128-byte rows of complex64) may run past it by 8 columns, which are neither read nor written (Past<N, T, E>::cut).
 #ifndef NSL_FULL_LINES
 #define NSL_FULL_LINES 1
 #endif
 
 // First-stage slot j of a pencil transform along y or z reads elements j*NB .. j*NB+NB-1 (NB = N / first radix).
 // True when none of them is a retained wavenumber: the slot is zero and needs no load.
+template <int N, int T, class E> struct Past {
+  static constexpr bool cut = trunc_geo<N>().hp % T != 0;
+  static_assert(!cut || !std::is_same<E, double2>::value, "FP64 tiles cover the row pitch exactly");
+  static_assert(!cut || T == 16, "only 16-line FP32 tiles run past the row pitch");
+};
 template <int N> __device__ __forceinline__ constexpr bool zero_slot(int j) {
   constexpr Geo g = trunc_geo<N>();
   constexpr int NB = N / FirstR<typename Sched<N>::L>::R;
@@ -174,7 +180,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const E *__restrict__ X1
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
-  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
+  constexpr bool cut = Past<N, T, E>::cut;
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
   const E *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const E *__restrict__ X1
   // every column of the tile is stored, padding kx >= hx included (its inputs are zero, so it stores zeros): whole
   // 128-byte lines, never a partial line
   auto store = [&](int n, int t, double2 v) {
-    if (NSL_FULL_LINES || kx0 + t < g.hx) stc(Ob[n * g.hp + t], v);
+    if ((NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) stc(Ob[n * g.hp + t], v);
   };
   fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
 }
@@ -216,7 +222,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const E *__restrict__ X3
   }
   extern __shared__ double2 lds[];
   constexpr Geo g = trunc_geo<N>();
-  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
+  constexpr bool cut = Past<N, T, E>::cut;
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
   const E *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const E *__restrict__ X3
   };
   auto store = [&](int n, int t, double2 v) {   // whole tiles, as in the y-inverse
     const int cy = compact_of(n, g);
-    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx)) stc(Ob[cy * g.hp + t], v);
+    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) stc(Ob[cy * g.hp + t], v);
   };
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }

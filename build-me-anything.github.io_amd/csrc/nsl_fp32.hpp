@@ -1,7 +1,11 @@
 // Passes of an FP32 solve: the spectral arrays in complex64 (float2) in HBM, the arithmetic in registers in FP64.
-// The same size-specialised rhs kernels (nsl_fastk.hpp) instantiated with storage E = float2, at the pencil tile
-// fast_T(N), with the z-forward fused into the update and one x-pass form per N. Included by nslab_hip.hip after
-// nsl_tune.hpp. An FP32 solve is a single solve (never a pool slot), so every launch has gridDim.z = 1 and sb = 0.
+// The same size-specialised rhs kernels (nsl_fastk.hpp) instantiated with storage E = float2, with the z-forward
+// fused into the update and one x-pass form per N. The y passes take 16-line tiles from 96^3 to 384^3: a row of 16
+// complex64 is one 128-byte line, where 8 lines read half lines (at 256^3, 8 lines fetched as many bytes as FP64:
+// y-forward 137 us, 16 lines 91 us; y-inverse 229 and 198 us). The z-inverse and the fused update keep the FP64
+// tiles: with 16 lines they ran slower (256^3: 171 against 112 us, one wave per SIMD at 256 registers; the update
+// 10 % slower). Included by nslab_hip.hip after nsl_tune.hpp. An FP32 solve is a single solve (never a pool slot),
+// so every launch has gridDim.z = 1 and sb = 0.
 //
 // The reference's FP32 mode (nslab_gpu.py:56-65) holds every array in complex64/float32 and transforms in single
 // precision. Here every array that lives in HBM between passes is complex64, so the state, the RK4 registers and
@@ -24,17 +28,20 @@ static constexpr long long xnl32_workgroups(int N) {
        : ((long long)N * (N / 2) + fast_LP(N) - 1) / fast_LP(N);
 }
 
+// Tile of the FP32 y passes: the whole row pitch up to 64^3 (as FP64), 16 lines to 384^3, 8 above (16 lines at
+// N >= 512 need 128+ KB of LDS per workgroup).
+static constexpr int fast_T32(int N) { return N <= 64 ? fast_T(N) : N <= 384 ? 16 : 8; }
+
 template <int N> struct P32 {
-  static constexpr int T = fast_T(N);
+  static constexpr int T = fast_T32(N), TZ = fast_T(N);   // y passes; z-inverse
   static constexpr Geo g = fast::trunc_geo<N>();
-  static constexpr int ntile = (g.hx + T - 1) / T;
-  static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
+  static constexpr int ntile = (g.hx + T - 1) / T, ntile_z = (g.hx + TZ - 1) / TZ;
 
   // P1: u and omega of U along z (six fields into X1)
   static int zinv(const Xform &x, const float2 *U, float2 *X1, const Ctl *ctl) {
-    const int lds = N * T * (int)sizeof(double2);
-    allow_lds(fast::kf_zinv_uw<N, T, float2>, lds);
-    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T, float2>), dim3((unsigned)(ntile * 2 * g.by)), dim3(fast::ZinvNT<N>::v), lds,
+    const int lds = N * TZ * (int)sizeof(double2);
+    allow_lds(fast::kf_zinv_uw<N, TZ, float2>, lds);
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, TZ, float2>), dim3((unsigned)(ntile_z * 2 * g.by)), dim3(fast::ZinvNT<N>::v), lds,
                        x.st, U, X1, x.g, x.twN, ctl, (size_t)0);
     LAUNCH_CHECK();
     return NSL_OK;
@@ -87,18 +94,23 @@ template <int N> struct P32 {
     return NSL_OK;
   }
   // P5: z-forward fused with the stage-`stage` update
+  // workgroups of the fused update (one sum partial each)
+  using ZC = fast::ZuCfg<N>;
+  static constexpr int zu_blocks = (g.hx + ZC::T - 1) / ZC::T * g.by;
   template <int STAGE>
   static void zu_stage(const Xform &x, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
                        double *partials, Ctl *ctl) {
-    using C = fast::ZuCfg<N>;
-    const int lds = N * C::T * (int)sizeof(double2);
+    const int lds = N * ZC::T * (int)sizeof(double2);
     const double inv = 1.0 / ((double)N * N * N);
     allow_lds(fast::kf_zfwd_upd<N, STAGE, 0, float2>, lds);
-    hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE, 0, float2>), dim3((unsigned)fast_zu_blocks(x.g)), dim3(C::NT), lds,
+    hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE, 0, float2>), dim3((unsigned)zu_blocks), dim3(ZC::NT), lds,
                        x.st, X1, U, S, ACC, TS, x.g, x.twN, inv, partials, ctl, (size_t)0);
   }
+  // *nsum: the sum partials k_finalize adds
   static int zu(const Xform &x, int stage, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
-                double *partials, Ctl *ctl) {
+                double *partials, Ctl *ctl, int *nsum) {
+    static_assert(zu_blocks <= g.by * ((g.hx + 3) / 4), "sum partials fit their buffer");
+    *nsum = zu_blocks;
     switch (stage) {
       case 1: zu_stage<1>(x, X1, U, S, ACC, TS, partials, ctl); break;
       case 2: zu_stage<2>(x, X1, U, S, ACC, TS, partials, ctl); break;
@@ -132,6 +144,6 @@ static int yfwd32(const Xform &x, const float2 *X2, float2 *X1, const Ctl *ctl, 
   NSL_FP32_DISPATCH(yfwd(x, X2, X1, ctl, pmax, nblk))
 }
 static int zu32(const Xform &x, int stage, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
-                double *partials, Ctl *ctl) {
-  NSL_FP32_DISPATCH(zu(x, stage, X1, U, S, ACC, TS, partials, ctl))
+                double *partials, Ctl *ctl, int *nsum) {
+  NSL_FP32_DISPATCH(zu(x, stage, X1, U, S, ACC, TS, partials, ctl, nsum))
 }

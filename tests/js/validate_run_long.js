@@ -93,6 +93,42 @@ check('run.log opens with the reference\'s line shape', logRe.test(fs.readFileSy
   check('--compare against the reference-written TGV 32^3 series: within 1e-13', c.steps === g.series.t.length &&
     Object.values(c.compare).every(v => v < 1e-13), JSON.stringify(c));
 }
+// --fp32: the GPU runner's FP32 mode (complex64 storage): final.json and its health report say float32, the two
+// roundoff-limited rows are ungraded with the runner's label suffix, run.log's header ends with the precision
+{
+  const F32 = path.join(base, 'f32');
+  quiet(() => RL.runLong(args(F32, ['--fp32'])));
+  const f = JSON.parse(fs.readFileSync(path.join(F32, 'final.json'), 'utf8'));
+  const G = F.fp32_grading;
+  const rows = f.health.rows.filter(r => r[0].endsWith(G.label_suffix));
+  check('--fp32 final.json: precision float32 in the file and its health report', f.precision === 'float32' &&
+    f.health.precision === 'float32' && f.health.note.endsWith(G.note_fp32), f.precision + ' ' + f.health.precision);
+  check('--fp32: divergence and nonlinear-transfer rows ungraded', rows.length === 2 && rows.every(r => r[2] === 'N/A') &&
+    rows[0][0].startsWith('divergence L∞') && rows[1][0].startsWith('nonlinear energy transfer'), JSON.stringify(rows));
+  check('--fp32 run.log header ends with float32', /float32\n/.test(fs.readFileSync(path.join(F32, 'run.log'), 'utf8').split('\n')[0] + '\n'));
+  // the same case in FP64 (run A above): the same steps, E within float32 rounding but not equal
+  const n = Math.min(f.series.E.length, fa.series.E.length);
+  let w = 0;
+  for (let i = 0; i < n; i++) w = Math.max(w, Math.abs(f.series.E[i] - fa.series.E[i]) / fa.series.E[i]);
+  check('--fp32 tracks the FP64 run: E within 1e-6, not identical', f.steps === fa.steps && w > 0 && w < 1e-6, w.toExponential(2));
+  check('--fp32 snapshot divergence at float32 roundoff, FP64 at FP64 roundoff',
+    f.snapshots[1].divMax > 1e-10 && f.snapshots[1].divMax < 1e-4 && fa.snapshots[1].divMax < 1e-12,
+    `${f.snapshots[1].divMax.toExponential(2)} ${fa.snapshots[1].divMax.toExponential(2)}`);
+}
+// --fp32 --compare against an FP32 run the reference's GPU runner wrote (tubes 128^3, Re 707, the first 40 steps of
+// research/nslab/expl-tubes-Re707-N128-fp32-gpu, tests/golden/archive_series.json), wrapped in the final.json schema
+{
+  const A32 = JSON.parse(fs.readFileSync(path.join(__dirname, '..', 'golden', 'archive_series.json'), 'utf8')).runs
+    .find(r => r.run === 'expl-tubes-Re707-N128-fp32-gpu');
+  const ser = A32.series, n = ser.t.length;
+  const wrapped = path.join(base, 'ref32_final.json');
+  fs.writeFileSync(wrapped, JSON.stringify({ case: A32.case, precision: A32.precision, t: ser.t[n - 1], steps: n - 1, series: ser,
+    final: false }));
+  const c = quiet(() => RL.runLong(RL.parseArgs(['--N', '128', '--Re', '707', '--ic', 'tubes', '--cfl', '0.4', '--tEnd', '16',
+    '--fp32', '--no-peak-track', '--out', path.join(base, 'r32'), '--compare', wrapped])));
+  check('--fp32 --compare against the reference FP32 tubes 128^3: E, Z, eps, max|ω| within 2e-6', c.steps === n &&
+    ['t', 'E', 'Z', 'eps', 'omMax'].every(k => c.compare[k] < 2e-6), JSON.stringify(c));
+}
 fs.rmdirSync(base, { recursive: true });
 console.log(fails ? `${fails} FAILED` : 'ALL PASS');
 process.exit(fails ? 1 : 0);
