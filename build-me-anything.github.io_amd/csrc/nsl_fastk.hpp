@@ -342,7 +342,12 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
   constexpr Geo g = trunc_geo<N>();
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
-  const int tile = blockIdx.x % ntile, cy = blockIdx.x / ntile, kx0 = tile * T;
+  // A tile narrower than a 128-byte line (4 lines at 1024^3, every FP32 tile) shares each line with its neighbour
+  // tile: dealt to the XCDs in contiguous logical ranges, the two halves of a line are read through one L2 (1024^3:
+  // update 13.5 -> 12.0 ms). Whole-line tiles keep the plain order (0.5-2 % faster at 256^3 and 512^3). The partial
+  // sums are indexed by the logical tile, so the summation order is the same either way.
+  const int bid = T * (int)sizeof(E) < 128 ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int tile = bid % ntile, cy = bid / ntile, kx0 = tile * T;
   const E *Ib = X4 + (size_t)cy * g.hp + kx0;
   double2 Nl[3][R];
 #pragma unroll
@@ -385,7 +390,7 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
       upd_apply<STAGE>(in[q], m, g.D, kx, ky, kwv(n, N), h, Nl[0][j], Nl[1][j], Nl[2][j], S, ACC, TS, nu, dt, acc);
     }
   }
-  block_sum<8, NT>(acc, reinterpret_cast<double *>(lds), partials + (size_t)blockIdx.x * 8);
+  block_sum<8, NT>(acc, reinterpret_cast<double *>(lds), partials + (size_t)bid * 8);
 }
 
 // LDS layout of the x pass (interleaved measured faster than line-major on gfx950).
@@ -787,10 +792,28 @@ template <int N, int V = 0> struct XnlStaged {
   // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
   // (three waves per SIMD, 168 registers, spills 95 at 192^3 and 111 at 384^3)
   static constexpr int MINW = NT == 128 ? NSL_XNL_BIG_MINW : NT > 64 ? 2 : 1;
-  // one-wave groups hold the twiddles of all five transforms in registers (TwCache); multi-wave groups read them
-  // from the table at each stage: at 1024^3 (8.8.2.8) the cache is the difference between 248 and 300+ registers
-  // (two waves per SIMD or one), at 192^3 it spills 14 registers
+  // one-wave groups hold the twiddles of all five transforms in registers (TwCache); multi-wave groups read them at
+  // each stage from a copy of the table entries their schedule uses, made in LDS when the workgroup starts
+  // (TwLen): at 1024^3 (8.8.2.8) a register cache is the difference between 248 and 300+ registers (two waves per
+  // SIMD or one), at 192^3 it spills 14 registers. Reading the global table instead costs a vector-memory wait in
+  // every stage, and vmcnt retires in issue order, so each such wait also waited for the staging loads of the next
+  // line in flight: the 1024^3 x pass took 18.1 ms that way, 16.9 ms with the LDS copy (and 228 instead of 248
+  // registers; the 384^3 staged form no longer spills).
   static constexpr bool TWC = NT == 64;
+};
+// Table entries a radix schedule reads (largest index + 1): stage (R, S) reads tw[j * base] for the powers j that
+// twiddles<R> takes from the table and base = (bb / S) * S, bb < N / R (the last stage reads none).
+template <int N, int S, class L> struct TwLen { static constexpr int v = 1; };
+template <int N, int S, int R0, int... Rs> struct TwLen<N, S, RList<R0, Rs...>> {
+  static constexpr int jmax() {
+    int m = 0;
+    for (int j = 1; j < R0; j++)
+      if (tw_from_table(j)) m = j;
+    return m;
+  }
+  static constexpr int mine = S * R0 == N ? 1 : jmax() * (((N / R0 - 1) / S) * S) + 1;
+  static constexpr int rest = TwLen<N, S * R0, RList<Rs...>>::v;
+  static constexpr int v = mine > rest ? mine : rest;
 };
 template <int N, int V = 0>
 __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
@@ -828,11 +851,19 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
       }
     }
   };
+  // multi-wave groups: the table entries of the schedule in LDS (copies, so the same values as the table)
+  constexpr int TLEN = C::TWC ? 1 : TwLen<N, 1, SCH>::v;
+  static_assert(TLEN <= N, "twiddle indices stay inside the table");
+  __shared__ double2 twl[TLEN];
+  if constexpr (!C::TWC) {
+    for (int i = tid; i < TLEN; i += NT) twl[i] = tw[i];
+    __syncthreads();
+  }
   stage_line(0, blockIdx.x * RPW);
   using TWC = typename std::conditional<C::TWC, TwCache<N, RPW, NT, LY, SCH>, const double2 *>::type;
   TWC twc;
   if constexpr (C::TWC) twc.load(tw);
-  else twc = tw;
+  else twc = twl;
   // first stage: radix R0, K0 butterflies per thread (slot (k, j) is element bb + j*NB0 of line t)
   using F0 = FirstR<SCH>;
   constexpr int R0 = F0::R, NB0 = N / R0, K0 = NB0 * RPW / NT;
