@@ -69,15 +69,22 @@ static void kf_zinv_launch(const Xform &x, const double2 *U, double2 *X1, int no
 }
 
 template <int N, int T>
-static int fast_zinv_nt(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
+static int fast_zinv_nt(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt,
+                        const Fin &fin) {
   const int lds = N * T * (int)sizeof(double2);
   if (mode == 1 && nout == 6 && N <= NSL_ZINV_UW_MAXN) {   // the rhs: u and omega from five transforms per tile
     // (4-line tiles at 1024^3, two workgroups per CU: 12.1 against 8.8 ms)
+    if (fin.nfin > 0 && (ctl == nullptr || fast::ZinvFin<N, T>::CH == 0))
+      return fail(NSL_ERR_STATE, "no rhs booking in this z-inverse");
     allow_lds(fast::kf_zinv_uw<N, T>, lds);
-    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by), 1, bt.nz), dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl, bt.sb);
+    // the booking workgroup writes the control block: a launch with fin.nfin > 0 comes from a solver's own rhs
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, T>), dim3((unsigned)(fast_ntile(x) * 2 * x.g.by + (fin.nfin > 0 ? 1 : 0)), 1, bt.nz),
+                       dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, x.twN, const_cast<Ctl *>(ctl), bt.sb, fin.partials,
+                       fin.nfin, fin.fmode);
     LAUNCH_CHECK();
     return NSL_OK;
   }
+  if (fin.nfin > 0) return fail(NSL_ERR_STATE, "no rhs booking in this z-inverse");
   if (mode == 0) kf_zinv_launch<N, T, 0>(x, U, X1, nout, gc, ctl, bt);
   else if (mode == 1) kf_zinv_launch<N, T, 1>(x, U, X1, nout, gc, ctl, bt);
   else kf_zinv_launch<N, T, 2>(x, U, X1, nout, gc, ctl, bt);
@@ -115,9 +122,10 @@ static int fast_zfwd_nt(const Xform &x, const double2 *X4, double2 *OUT, int nco
 }
 // The pencil passes at the solver's tile width x.fT (fast_T(N), or alt_T(N) when the autotuner picked it).
 template <int N>
-static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
-  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_zinv_nt<N, alt_T(N)>(x, U, X1, mode, nout, gc, ctl, bt);
-  return fast_zinv_nt<N, fast_T(N)>(x, U, X1, mode, nout, gc, ctl, bt);
+static int fast_zinv_n(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt,
+                       const Fin &fin) {
+  if constexpr (alt_T(N) > 0) if (x.fT == alt_T(N)) return fast_zinv_nt<N, alt_T(N)>(x, U, X1, mode, nout, gc, ctl, bt, fin);
+  return fast_zinv_nt<N, fast_T(N)>(x, U, X1, mode, nout, gc, ctl, bt, fin);
 }
 template <int N>
 static int fast_yinv_n(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
@@ -277,8 +285,9 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
     default: return fail(NSL_ERR_STATE, "no size-specialised kernels for this N"); \
   }
 
-static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt) {
-  NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl, bt)
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt,
+                              const Fin &fin) {
+  NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl, bt, fin)
 }
 static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl, bt)

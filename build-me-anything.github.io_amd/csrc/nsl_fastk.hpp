@@ -95,17 +95,33 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_zinv(const double2 *__restric
 template <int N> struct ZinvMinWaves { static constexpr int v = N > 512 ? NSL_ZINV_BIG_MINW : N == 512 ? 4 : 1; };
 // Threads of the z-inverse: at N = 256 (16 x 16, 16 outputs per butterfly) 128 threads, one butterfly each.
 template <int N> struct ZinvNT { static constexpr int v = N == 256 ? 128 : PassNT<N>::v; };
+// nfin > 0: one workgroup more than the tiles need, the grid's first (dispatched first, so its single-workgroup
+// sums overlap the tiles rather than trailing them), books the previous rhs if its fused update left that pending
+// (fin_rhs: its nfin partials, diagnostics into stage fmode), in this launch instead of a kernel of its own; the
+// tiles' workgroups never read what it writes. Its sums use the tile's LDS (N T complex) as scratch.
+template <int N, int T> struct ZinvFin {
+  static constexpr int SCR = N * T * 2 / NSL_NT;   // doubles per virtual thread the LDS holds
+  static constexpr int CH = SCR >= 4 ? 4 : SCR >= 2 ? 2 : SCR >= 1 ? 1 : 0;
+};
 template <int N, int T, class E = double2>
 __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(const E *__restrict__ U, E *__restrict__ X, const Geo,
-                                                      const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+                                                      const double2 *__restrict__ tw, Ctl *__restrict__ ctl, size_t sb,
+                                                      const double *__restrict__ partials, int nfin, int fmode) {
   U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   extern __shared__ double2 lds[];
+  const int first = nfin > 0 ? 1 : 0, ntiles = (int)gridDim.x - first;
+  if ((int)blockIdx.x < first) {
+    if constexpr (ZinvFin<N, T>::CH > 0)
+      if (ctl->pend != 0)
+        fin_rhs<ZinvNT<N>::v, ZinvFin<N, T>::CH>(at_slot(partials, sb), nfin, fmode, nullptr, ctl, (double *)lds);
+    return;
+  }
   constexpr Geo g = trunc_geo<N>();
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
   using OR = OutRegs<N, T, ZinvNT<N>::v>;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bid = xcd_remap((int)blockIdx.x - first, ntiles);
   const int tile = bid % ntile, rest = bid / ntile, type = rest & 1, cy = rest >> 1, kx0 = tile * T;
   const double ky = kwv(full_of(cy, g), N);
   const size_t ubase = (size_t)cy * g.hp + kx0;
@@ -391,6 +407,9 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
     }
   }
   block_sum<8, NT>(acc, reinterpret_cast<double *>(lds), partials + (size_t)bid * 8);
+  // the booking of these partials (fin_rhs) is left to the next launch that reads them: the next rhs's z-inverse,
+  // the next step's k_begin_step, or a k_finalize at the end of the steps enqueued
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->pend = 1;
 }
 
 // LDS layout of the x pass (interleaved measured faster than line-major on gfx950).

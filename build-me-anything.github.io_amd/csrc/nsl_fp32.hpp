@@ -38,11 +38,14 @@ template <int N> struct P32 {
   static constexpr int ntile = (g.hx + T - 1) / T, ntile_z = (g.hx + TZ - 1) / TZ;
 
   // P1: u and omega of U along z (six fields into X1)
-  static int zinv(const Xform &x, const float2 *U, float2 *X1, const Ctl *ctl) {
+  // fin: the previous rhs's booking, done by one extra workgroup (kf_zinv_uw)
+  static int zinv(const Xform &x, const float2 *U, float2 *X1, Ctl *ctl, const Fin &fin) {
     const int lds = N * TZ * (int)sizeof(double2);
+    static_assert(fast::ZinvFin<N, TZ>::CH > 0, "the z-inverse's LDS holds the booking's scratch");
     allow_lds(fast::kf_zinv_uw<N, TZ, float2>, lds);
-    hipLaunchKernelGGL((fast::kf_zinv_uw<N, TZ, float2>), dim3((unsigned)(ntile_z * 2 * g.by)), dim3(fast::ZinvNT<N>::v), lds,
-                       x.st, U, X1, x.g, x.twN, ctl, (size_t)0);
+    hipLaunchKernelGGL((fast::kf_zinv_uw<N, TZ, float2>), dim3((unsigned)(ntile_z * 2 * g.by + (fin.nfin > 0 ? 1 : 0))),
+                       dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl, (size_t)0, fin.partials, fin.nfin,
+                       fin.fmode);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -137,7 +140,7 @@ template <int N> struct P32 {
     default: return fail(NSL_ERR_STATE, "no FP32 kernels for N = " + std::to_string(x.g.N)); \
   }
 
-static int zinv32(const Xform &x, const float2 *U, float2 *X1, const Ctl *ctl) { NSL_FP32_DISPATCH(zinv(x, U, X1, ctl)) }
+static int zinv32(const Xform &x, const float2 *U, float2 *X1, Ctl *ctl, const Fin &fin) { NSL_FP32_DISPATCH(zinv(x, U, X1, ctl, fin)) }
 static int yinv32(const Xform &x, const float2 *X1, float2 *X2, const Ctl *ctl) { NSL_FP32_DISPATCH(yinv(x, X1, X2, ctl)) }
 static int xnl32(const Xform &x, float2 *X2, double *pmax, Ctl *ctl, int *nred) { NSL_FP32_DISPATCH(xnl(x, X2, pmax, ctl, nred)) }
 static int yfwd32(const Xform &x, const float2 *X2, float2 *X1, const Ctl *ctl, const double *pmax, int nblk) {

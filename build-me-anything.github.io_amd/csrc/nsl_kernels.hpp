@@ -19,7 +19,7 @@ namespace nsl {
 // One per solver, in device memory.
 struct Ctl {
   int active;            // the step in flight executes (0: every kernel of the step returns at once)
-  int record;            // unused padding kept for alignment
+  int pend;              // the fused update of the last rhs left its partial sums unbooked (see fin_rhs)
   long long nDone;       // steps done in the current run call
   long long nTarget;     // steps requested
   long long nChunk;      // records written since the last chunk start
@@ -487,18 +487,52 @@ __global__ __launch_bounds__(NSL_NT) void k_update(const double2 *__restrict__ N
 // Sums the per-block partials in a fixed order and books the rhs diagnostics. mode 0..3: stage mode+1 of the step
 // in flight; mode 4: `last` (rhs of the current state) and, when rec != null, the end of the step: budgets
 // (nslab.js:326-329), t += dt, step++, one StepRecDev.
-__global__ __launch_bounds__(NSL_NT) void k_finalize(const double *__restrict__ partials, int nblk, int mode,
-                                                      StepRecDev *__restrict__ rec, Ctl *__restrict__ ctl, size_t sb) {
-  __shared__ double scratch[8 * NSL_NT];
-  __shared__ double tot[8];
-  partials = at_slot(partials, sb); rec = at_slot(rec, sb); ctl = at_slot(ctl, sb);
-  if (ctl->active == 0) return;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < nblk; b += NSL_NT)
+// Run by one workgroup of NT threads (any NT), which plays NSL_NT virtual threads: virtual thread v adds rows v,
+// v + NSL_NT, ... and the rows' totals follow block_sum<8, NSL_NT>'s tree, CH components at a time, so every
+// caller gets the same bits whatever its own workgroup size. scratch: CH * NSL_NT doubles of LDS.
+template <int NT, int CH = 4>
+__device__ __forceinline__ void fin_rhs(const double *__restrict__ partials, int nblk, int mode,
+                                        StepRecDev *__restrict__ rec, Ctl *__restrict__ ctl, double *scratch) {
+  static_assert(CH == 1 || CH == 2 || CH == 4 || CH == 8, "components per round divide 8");
+  constexpr int V = NSL_NT, KV = (V + NT - 1) / NT, P2 = pow2_ceil(V);
+  double acc[KV][8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] += partials[(size_t)b * 8 + i];
-  block_sum<8>(acc, scratch, tot);
+  for (int k = 0; k < KV; k++) {
+    const int v = threadIdx.x + k * NT;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[k][i] = 0.0;
+    if (v < V)
+      for (int b = v; b < nblk; b += V)
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[k][i] += partials[(size_t)b * 8 + i];
+  }
+  double tot[8];
+#pragma unroll
+  for (int h = 0; h < 8 / CH; h++) {
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+      const int v = threadIdx.x + k * NT;
+      if (v < V)
+#pragma unroll
+        for (int i = 0; i < CH; i++) scratch[i * V + v] = acc[k][CH * h + i];
+    }
+    __syncthreads();
+    for (int w = P2 / 2; w > 0; w >>= 1) {
+#pragma unroll
+      for (int k = 0; k < KV; k++) {
+        const int v = threadIdx.x + k * NT;
+        if (v < w && v + w < V)
+#pragma unroll
+          for (int i = 0; i < CH; i++) scratch[i * V + v] += scratch[i * V + v + w];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < CH; i++) tot[CH * h + i] = scratch[i * V];
+  }
   if (threadIdx.x != 0) return;
+  ctl->pend = 0;
   const double nu = ctl->nu;
   double *d = ctl->diag[mode];
   const double Z = tot[3];
@@ -526,9 +560,26 @@ __global__ __launch_bounds__(NSL_NT) void k_finalize(const double *__restrict__ 
   }
 }
 
-// Start of a step: the stop predicate of run() (nslab.js:334) and dt of this step (nslab.js:314).
-__global__ void k_begin_step(Ctl *ctl, size_t sb) {
-  ctl = at_slot(ctl, sb);
+// The rhs's bookkeeping as a kernel of its own. need_pend = 0: right after an update that does not defer it (the
+// generic and two-kernel paths); need_pend = 1: whatever a fused update left pending (the end of a chunk of steps,
+// a rhs outside a step), nothing otherwise.
+__global__ __launch_bounds__(NSL_NT) void k_finalize(const double *__restrict__ partials, int nblk, int mode,
+                                                      StepRecDev *__restrict__ rec, Ctl *__restrict__ ctl, size_t sb,
+                                                      int need_pend) {
+  __shared__ double scratch[4 * NSL_NT];
+  partials = at_slot(partials, sb); rec = at_slot(rec, sb); ctl = at_slot(ctl, sb);
+  if (ctl->active == 0 || (need_pend && ctl->pend == 0)) return;
+  fin_rhs<NSL_NT>(partials, nblk, mode, rec, ctl, scratch);
+}
+
+// Start of a step: first the end of the previous step, if a fused update left it pending (nfin partials, mode 4,
+// one StepRecDev), then the stop predicate of run() (nslab.js:334) and dt of this step (nslab.js:314).
+__global__ __launch_bounds__(NSL_NT) void k_begin_step(Ctl *ctl, const double *__restrict__ partials, int nfin,
+                                                       StepRecDev *__restrict__ rec, size_t sb) {
+  __shared__ double scratch[4 * NSL_NT];
+  ctl = at_slot(ctl, sb); partials = at_slot(partials, sb); rec = at_slot(rec, sb);
+  if (nfin > 0 && ctl->pend != 0) fin_rhs<NSL_NT>(partials, nfin, 4, rec, ctl, scratch);
+  if (threadIdx.x != 0) return;
   const bool act = ctl->nDone < ctl->nTarget && !(ctl->hasTEnd && ctl->t >= ctl->tEnd - 1e-12);
   ctl->active = act ? 1 : 0;
   if (act) {

@@ -161,8 +161,16 @@ struct Batch {
   unsigned nz = 1;
   size_t sb = 0;
 };
+// The booking of the previous rhs that a fused update left pending (Ctl::pend), done by one extra workgroup of the
+// next rhs's z-inverse: the update's partials, their number (0: nothing to book here) and the diagnostics stage.
+struct Fin {
+  const double *partials = nullptr;
+  int nfin = 0;
+  int fmode = 0;
+};
 
-static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt);
+static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl, const Batch &bt,
+                              const Fin &fin);
 static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt);
 static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
                               const double *pmax, int nblk);
@@ -204,8 +212,9 @@ static int need_single(const Batch &bt) {
 }
 
 static int zinv(const Xform &x, const double2 *U, double2 *X1, int mode, int nout, int gc, const Ctl *ctl,
-                const Batch &bt = Batch()) {
-  if (x.fast) return fast_dispatch_zinv(x, U, X1, mode, nout, gc, ctl, bt);
+                const Batch &bt = Batch(), const Fin &fin = Fin()) {
+  if (x.fast) return fast_dispatch_zinv(x, U, X1, mode, nout, gc, ctl, bt, fin);
+  if (fin.nfin > 0) return fail(NSL_ERR_STATE, "a pending rhs booking needs the fused update's z-inverse");
   TRY(need_single(bt));
   return x.tl.TNp == 4096 ? zinv_t<4096>(x, U, X1, mode, nout, gc, ctl) : zinv_t<8192>(x, U, X1, mode, nout, gc, ctl);
 }
@@ -473,11 +482,32 @@ static void diag_from(const double *d, nsl_rhsdiag &o) {
   o.E = d[0]; o.Z = d[1]; o.P = d[2]; o.eps = d[3]; o.Tnl = d[4]; o.Pspec = d[5]; o.omMax = d[6]; o.uMax = d[7];
 }
 
-// One rhs of an FP32 solve's S32 (onS) or T32, with the fused stage-`stage` update.
-static int rhs_enqueue32(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev *rec) {
+// Rows of partial sums a fused update leaves for the next launch to book (Ctl::pend), or 0 when the solver's rhs
+// books them at once (the generic and two-kernel paths).
+static int pending_rows(const nsl_solver *s) { return s->fp32 || fast_has_zu(s->x) ? fast_zu_blocks(s->x.g) : 0; }
+// The booking of the rhs before this one, when its fused update left it pending: done by the z-inverse of this rhs.
+static Fin pending_fin(const nsl_solver *s, int prev_fmode) {
+  Fin f;
+  f.partials = s->partials; f.nfin = pending_rows(s); f.fmode = prev_fmode;
+  return f;
+}
+// Books whatever a fused update left pending (mode 4: the rhs of the current state; with rec, the end of a step):
+// after the last step enqueued, and after a rhs outside a step. Nothing pending, nothing done.
+static int flush_pending(nsl_solver *s, StepRecDev *rec) {
+  const int nfin = pending_rows(s);
+  if (nfin == 0) return NSL_OK;
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, s->bt.nz), dim3(NSL_NT), 0, s->st, s->partials, nfin, 4, rec, s->ctl,
+                     s->bt.sb, 1);
+  LAUNCH_CHECK();
+  return NSL_OK;
+}
+
+// One rhs of an FP32 solve's S32 (onS) or T32, with the fused stage-`stage` update (its sums booked later, as in
+// rhs_enqueue); fin: the previous rhs's booking.
+static int rhs_enqueue32(nsl_solver *s, bool onS, int stage, const Fin &fin) {
   const Xform &x = s->x;
   const float2 *U = onS ? s->S32 : s->T32;
-  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv32(x, U, s->X1_32, s->ctl)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv32(x, U, s->X1_32, s->ctl, fin)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv32(x, s->X1_32, s->X2_32, s->ctl)); TRY(prof_end(s));
   int nred = 0;
   TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl32(x, s->X2_32, s->pmax, s->ctl, &nred)); TRY(prof_end(s));
@@ -486,32 +516,30 @@ static int rhs_enqueue32(nsl_solver *s, bool onS, int stage, int fmode, StepRecD
   TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
   TRY(zu32(x, stage, s->X1_32, U, s->S32, s->ACC32, s->T32, s->partials, s->ctl, &nsum));
   TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_FINALIZE));
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NSL_NT), 0, s->st, s->partials, nsum, fmode, rec, s->ctl, (size_t)0);
-  LAUNCH_CHECK();
-  TRY(prof_end(s));
   return NSL_OK;
 }
 
-// One rhs of S (onS) or T with the fused stage-`stage` update; fmode selects where k_finalize books the sums.
-static int rhs_enqueue(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev *rec) {
-  if (s->fp32) return rhs_enqueue32(s, onS, stage, fmode, rec);
+// One rhs of S (onS) or T with the stage-`stage` update; fmode selects where the sums are booked (diag[fmode]).
+// The fused update leaves its sums pending (Ctl::pend) and they are booked by the next launch that follows: the next
+// rhs's z-inverse (fin, one extra workgroup), the next step's k_begin_step or flush_pending, instead of a
+// single-workgroup kernel after every rhs. The generic and two-kernel paths book them at once (k_finalize).
+static int rhs_enqueue(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev *rec, const Fin &fin) {
+  if (s->fp32) return rhs_enqueue32(s, onS, stage, fin);
   const Xform &x = s->x;
   const Batch &bt = s->bt;
   double2 *U = onS ? s->S : s->T;
-  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt)); TRY(prof_end(s));
+  TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt, fin)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
   int nred = 0;
   TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, &nred, bt)); TRY(prof_end(s));
   TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt, s->pmax, nred)); TRY(prof_end(s));
-  int nsum = s->nblk;   // partials the finalize sums
   if (fast_has_zu(x)) {
     // z-forward and update in one pass: the nonlinear term stays on chip (timed as the update of this stage)
     TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
     TRY(fast_dispatch_zu(x, stage, s->X1, U, s->S, s->ACC, s->T, s->partials, s->ctl, bt));
     TRY(prof_end(s));
-    nsum = fast_zu_blocks(x.g);
-  } else {
+    return NSL_OK;
+  }
   TRY(prof_begin(s, NSL_PROF_ZFWD)); TRY(zfwd(x, s->X1, s->NL, 3, s->ctl, bt)); TRY(prof_end(s));
   dim3 g((unsigned)s->nblk, 1, bt.nz), b(NSL_NT);
   TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));
@@ -523,23 +551,24 @@ static int rhs_enqueue(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev
   }
   LAUNCH_CHECK();
   TRY(prof_end(s));
-  }
   TRY(prof_begin(s, NSL_PROF_FINALIZE));
-  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bt.nz), dim3(NSL_NT), 0, s->st, s->partials, nsum, fmode, rec, s->ctl, bt.sb);
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bt.nz), dim3(NSL_NT), 0, s->st, s->partials, s->nblk, fmode, rec, s->ctl, bt.sb, 0);
   LAUNCH_CHECK();
   TRY(prof_end(s));
   return NSL_OK;
 }
 
 // One RK4 step (nslab.js:312-333). Stage 1's rhs is the previous step's last rhs (identical input), already
-// booked in diag[4] with ACC and T prepared; the step ends with the rhs of the new state.
+// booked in diag[4] with ACC and T prepared; the step ends with the rhs of the new state. k_begin_step first books
+// the previous step's end if it is pending (budgets, t, step, one record).
 static int step_enqueue(nsl_solver *s) {
-  hipLaunchKernelGGL(k_begin_step, dim3(1, 1, s->bt.nz), dim3(1), 0, s->st, s->ctl, s->bt.sb);
+  hipLaunchKernelGGL(k_begin_step, dim3(1, 1, s->bt.nz), dim3(NSL_NT), 0, s->st, s->ctl, (const double *)s->partials,
+                     pending_rows(s), s->recs, s->bt.sb);
   LAUNCH_CHECK();
-  TRY(rhs_enqueue(s, false, 2, 1, nullptr));
-  TRY(rhs_enqueue(s, false, 3, 2, nullptr));
-  TRY(rhs_enqueue(s, false, 4, 3, nullptr));
-  TRY(rhs_enqueue(s, true, 1, 4, s->recs));
+  TRY(rhs_enqueue(s, false, 2, 1, nullptr, Fin()));
+  TRY(rhs_enqueue(s, false, 3, 2, nullptr, pending_fin(s, 1)));
+  TRY(rhs_enqueue(s, false, 4, 3, nullptr, pending_fin(s, 2)));
+  TRY(rhs_enqueue(s, true, 1, 4, s->recs, pending_fin(s, 3)));
   return NSL_OK;
 }
 
@@ -547,7 +576,8 @@ static int step_enqueue(nsl_solver *s) {
 static int rhs_current(nsl_solver *s) {
   hipLaunchKernelGGL(k_set_active, dim3(1), dim3(1), 0, s->st, s->ctl, 1, (size_t)0);
   LAUNCH_CHECK();
-  TRY(rhs_enqueue(s, true, 1, 4, nullptr));
+  TRY(rhs_enqueue(s, true, 1, 4, nullptr, Fin()));
+  TRY(flush_pending(s, nullptr));
   TRY(pull_ctl(s));
   diag_from(s->hctl->diag[4], s->last);
   return NSL_OK;
@@ -862,6 +892,7 @@ static int run_impl(nsl_solver *s, int64_t n, int has_tend, double tend, nsl_ste
     hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(1), 0, s->st, s->ctl, (size_t)0);
     LAUNCH_CHECK();
     for (int i = 0; i < chunk; i++) TRY(step_enqueue(s));
+    TRY(flush_pending(s, s->recs));   // the last step's end
     HIP_TRY(hipMemcpyAsync(s->hrecs, s->recs, chunk * sizeof(StepRecDev), hipMemcpyDeviceToHost, s->st));
     TRY(pull_ctl(s));
     const int got = (int)c->nChunk;
@@ -1104,6 +1135,7 @@ static int pool_launch_impl(nsl_pool *p, int steps) {
     if (p->graphed) HIP_TRY(hipGraphLaunch(p->gexec, p->st));
     else TRY(step_enqueue(p->lead));
   }
+  TRY(flush_pending(p->lead, p->lead->recs));   // every slot's last step's end
   hipLaunchKernelGGL(k_pool_gather, dim3(p->C), dim3(256), 0, p->st, (const char *)p->base, p->L.bytes, p->L.ctl,
                      p->L.recs, steps, p->dstage);
   LAUNCH_CHECK();
@@ -1239,6 +1271,12 @@ static int group_launch_impl(nsl_group *g, int steps) {
   }
   for (int i = 0; i < steps; i++) HIP_TRY(hipGraphLaunch(g->gexec, g->st));
   for (nsl_pool *p : g->pools) {
+    const int nfin = pending_rows(p->lead);   // every slot's last step's end
+    if (nfin > 0) {
+      hipLaunchKernelGGL(k_finalize, dim3(1, 1, p->C), dim3(NSL_NT), 0, g->st, (const double *)p->lead->partials, nfin, 4,
+                         p->lead->recs, p->lead->ctl, p->L.bytes, 1);
+      LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_pool_gather, dim3(p->C), dim3(256), 0, g->st, (const char *)p->base, p->L.bytes, p->L.ctl,
                        p->L.recs, steps, p->dstage);
     LAUNCH_CHECK();
