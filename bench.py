@@ -10,7 +10,7 @@ event pair around every launch, for the per-kernel averages (the events add gaps
 Printed: one JSON line with the throughput, the roofline of the dominant kernel (algorithmic bytes per launch over
 its mean launch time, from HIP events recorded on the solver's stream inside the second timed region), the whole-step
 roofline fraction of the truncated-storage traffic model, a 512^3 line, FP32-storage lines at the headline size
-and 512^3 (the GPU runner's fp32 mode: complex64 in HBM, FP64 arithmetic in registers), the CPU baseline (the oracle, a
+and 512^3 (the GPU runner's fp32 mode: complex64 in HBM, single-precision transforms), the CPU baseline (the oracle, a
 single-threaded C restatement of the reference algorithm, timed on this host on a bounded sample), and three lines
 from the Node host (js/serve.js): the batched ensemble (128 solves of uneven N in slot pools, one hipGraph per step),
 the ladder request loop (Poisson arrivals, admission by HBM footprint, p50/p99 latency, ladders per hour, against
@@ -68,7 +68,7 @@ def b_alg_step(N):
 
 
 def measure(N, steps, warmup, dt, label, fp32=False):
-    """fp32: an FP32 solve (complex64 storage, FP64 arithmetic in registers): 8 bytes per stored complex value."""
+    """fp32: an FP32 solve (complex64 storage, single-precision transforms): 8 bytes per stored complex value."""
     import capi
     import torch
     s = capi.Solver(N=N, Re=1600, ic="tgv", dt=dt, fp32=fp32)
@@ -400,13 +400,13 @@ def main():
                        "kernels": {n: {"avg_us": round(v["avg_us"], 2), "GBs": round(v["GBs"], 1)}
                                    for n, v in extra["kernels"].items()}}
     if not a.no_fp32:
-        # the GPU runner's FP32 mode: complex64 in HBM (half the bytes of every pass), FP64 arithmetic in registers
+        # the GPU runner's FP32 mode: complex64 in HBM (half the bytes of every pass), single-precision transforms
         f32 = [measure(a.N, a.steps, a.warmup, a.dt, "fp32", fp32=True)]
         if not a.no_512 and a.N != 512:
             f32.append(measure(512, max(4, a.steps // 4), 1, 0.002, "fp32 512^3", fp32=True))
         fp64_by_n = {a.N: main_r, 512: extra}
         out["fp32"] = [{
-            "workload": "Taylor-Green %d^3, FP32 storage (complex64), FP64 arithmetic in registers, dt=%g fixed"
+            "workload": "Taylor-Green %d^3, FP32 (complex64 storage, single-precision transforms, FP64 update), dt=%g fixed"
                         % (r["N"], a.dt if r["N"] == a.N else 0.002),
             "steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
             "vs_fp64": r["steps_per_s"] / fp64_by_n[r["N"]]["steps_per_s"] if fp64_by_n.get(r["N"]) else None,

@@ -18,15 +18,23 @@ namespace fast {
 #define NSL_FFT_CONTRACT _Pragma("clang fp contract(fast)")
 #endif
 
-template <int SIGN>
-__device__ __forceinline__ double2 mul_tw(double2 a, double2 w) {   // a * (w.x + i SIGN w.y)
+// The engine is generic in its complex type C: double2 (FP64 solves) or float2 (FP32 solves, which transform in
+// single precision as the reference's FP32 runner does). Sc<C> is the scalar type, mk<C> builds a value; every
+// expression below is the same for both types, so the double2 instantiation is the FP64 engine (the compiler may
+// pick the other product of an a*b - c*d to fuse, a last-bit difference within the 1e-13 parity bound).
+template <class C> using Sc = decltype(C{}.x);
+template <class C> __device__ __forceinline__ C mk(Sc<C> x, Sc<C> y) { C v; v.x = x; v.y = y; return v; }
+
+template <int SIGN, class C>
+__device__ __forceinline__ C mul_tw(C a, C w) {   // a * (w.x + i SIGN w.y)
   NSL_FFT_CONTRACT
-  const double c = w.x, d = SIGN * w.y;
-  return make_double2(a.x * c - a.y * d, a.x * d + a.y * c);
+  const Sc<C> c = w.x, d = SIGN * w.y;
+  return mk<C>(a.x * c - a.y * d, a.x * d + a.y * c);
 }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) {
   NSL_FFT_CONTRACT
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // Twiddles w^j, j = 1..R-1, of one butterfly, w = (cos, sin)(2 pi base / N) and w^j = tw[j * base]. w^1..w^3 and
@@ -36,76 +44,79 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 #ifndef NSL_TW_PRODUCTS
 #define NSL_TW_PRODUCTS 1
 #endif
-template <int R>
-__device__ __forceinline__ void twiddles(const double2 *__restrict__ tw, int base, double2 (&w)[R]) {
+template <int R, class C>
+__device__ __forceinline__ void twiddles(const C *__restrict__ tw, int base, C (&w)[R]) {
 #pragma unroll
   for (int j = 1; j < R; j++) {
     if (!NSL_TW_PRODUCTS || j < 4 || (j & 3) == 0) w[j] = tw[j * base];
     else w[j] = cmul(w[j & ~3], w[j & 3]);
   }
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <class C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
 // multiply by SIGN * i
-template <int SIGN>
-__device__ __forceinline__ double2 mul_si(double2 a) { return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x); }
+template <int SIGN, class C>
+__device__ __forceinline__ C mul_si(C a) { return SIGN > 0 ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x); }
 
 // In-register DFTs: y_j = sum_k x_k w^(jk), w = exp(SIGN 2 pi i / R), natural order in and out.
 template <int R, int SIGN> struct Dft;
 
 template <int SIGN> struct Dft<2, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
-    const double2 a = v[0], b = v[1];
+  template <class C> static __device__ __forceinline__ void run(C *v) {
+    const C a = v[0], b = v[1];
     v[0] = cadd(a, b); v[1] = csub(a, b);
   }
 };
 
 template <int SIGN> struct Dft<3, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
+  template <class C> static __device__ __forceinline__ void run(C *v) {
     NSL_FFT_CONTRACT
-    const double S3 = -SIGN * 0.8660254037844386;
-    const double2 a0 = v[0], a1 = v[1], a2 = v[2];
-    const double sr = a1.x + a2.x, si = a1.y + a2.y, ur = a0.x - 0.5 * sr, ui = a0.y - 0.5 * si;
-    const double vr = S3 * (a1.x - a2.x), vi = S3 * (a1.y - a2.y);
-    v[0] = make_double2(a0.x + sr, a0.y + si);
-    v[1] = make_double2(ur + vi, ui - vr);
-    v[2] = make_double2(ur - vi, ui + vr);
+    using F = Sc<C>;
+    const F S3 = -SIGN * (F)0.8660254037844386, half = (F)0.5;
+    const C a0 = v[0], a1 = v[1], a2 = v[2];
+    const F sr = a1.x + a2.x, si = a1.y + a2.y, ur = a0.x - half * sr, ui = a0.y - half * si;
+    const F vr = S3 * (a1.x - a2.x), vi = S3 * (a1.y - a2.y);
+    v[0] = mk<C>(a0.x + sr, a0.y + si);
+    v[1] = mk<C>(ur + vi, ui - vr);
+    v[2] = mk<C>(ur - vi, ui + vr);
   }
 };
 
 template <int SIGN> struct Dft<4, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
-    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
-    const double2 st3 = mul_si<SIGN>(t3);
+  template <class C> static __device__ __forceinline__ void run(C *v) {
+    const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]), t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+    const C st3 = mul_si<SIGN>(t3);
     v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
     v[1] = cadd(t1, st3); v[3] = csub(t1, st3);
   }
 };
 
 template <int SIGN> struct Dft<6, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
+  template <class C> static __device__ __forceinline__ void run(C *v) {
     // n = 2 n1 + n2, j = j1 + 3 j2: DFT3 over n1, twiddle w6^(j1 n2), DFT2 over n2
-    double2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
+    using F = Sc<C>;
+    C e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]};
     Dft<3, SIGN>::run(e);
     Dft<3, SIGN>::run(o);
-    o[1] = mul_tw<SIGN>(o[1], make_double2(0.5, 0.8660254037844386));    // w6
-    o[2] = mul_tw<SIGN>(o[2], make_double2(-0.5, 0.8660254037844386));   // w6^2
+    o[1] = mul_tw<SIGN>(o[1], mk<C>((F)0.5, (F)0.8660254037844386));    // w6
+    o[2] = mul_tw<SIGN>(o[2], mk<C>((F)-0.5, (F)0.8660254037844386));   // w6^2
 #pragma unroll
     for (int j1 = 0; j1 < 3; j1++) { v[j1] = cadd(e[j1], o[j1]); v[j1 + 3] = csub(e[j1], o[j1]); }
   }
 };
 
 template <int SIGN> struct Dft<8, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
+  template <class C> static __device__ __forceinline__ void run(C *v) {
     NSL_FFT_CONTRACT
-    const double H = 0.7071067811865476;
-    double2 a[4], b[4];
+    using F = Sc<C>;
+    const F H = (F)0.7071067811865476;
+    C a[4], b[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) { a[k] = cadd(v[k], v[k + 4]); b[k] = csub(v[k], v[k + 4]); }
     // b_k *= w8^k, w8 = exp(SIGN i pi/4)
-    b[1] = make_double2(H * (b[1].x - SIGN * b[1].y), H * (b[1].y + SIGN * b[1].x));
+    b[1] = mk<C>(H * (b[1].x - SIGN * b[1].y), H * (b[1].y + SIGN * b[1].x));
     b[2] = mul_si<SIGN>(b[2]);
-    b[3] = make_double2(H * (-b[3].x - SIGN * b[3].y), H * (-b[3].y + SIGN * b[3].x));
+    b[3] = mk<C>(H * (-b[3].x - SIGN * b[3].y), H * (-b[3].y + SIGN * b[3].x));
     Dft<4, SIGN>::run(a);
     Dft<4, SIGN>::run(b);
 #pragma unroll
@@ -114,30 +125,31 @@ template <int SIGN> struct Dft<8, SIGN> {
 };
 
 template <int SIGN> struct Dft<16, SIGN> {
-  static __device__ __forceinline__ void run(double2 *v) {
+  template <class C> static __device__ __forceinline__ void run(C *v) {
     // n = 4 n1 + n2, j = j1 + 4 j2: DFT4 over n1, twiddle w16^(j1 n2), DFT4 over n2
-    const double C1 = 0.9238795325112867, S1 = 0.3826834323650898, H = 0.7071067811865476;
-    double2 A[4][4];   // A[n2][j1]
+    using F = Sc<C>;
+    const F C1 = (F)0.9238795325112867, S1 = (F)0.3826834323650898, H = (F)0.7071067811865476;
+    C A[4][4];   // A[n2][j1]
 #pragma unroll
     for (int n2 = 0; n2 < 4; n2++) {
-      double2 c[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      C c[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
       Dft<4, SIGN>::run(c);
 #pragma unroll
       for (int j1 = 0; j1 < 4; j1++) A[n2][j1] = c[j1];
     }
     // twiddles w16^(j1 n2) = exp(SIGN i 2 pi j1 n2 / 16) for j1, n2 in 1..3
-    A[1][1] = mul_tw<SIGN>(A[1][1], make_double2(C1, S1));
-    A[1][2] = mul_tw<SIGN>(A[1][2], make_double2(H, H));
-    A[1][3] = mul_tw<SIGN>(A[1][3], make_double2(S1, C1));
-    A[2][1] = mul_tw<SIGN>(A[2][1], make_double2(H, H));
+    A[1][1] = mul_tw<SIGN>(A[1][1], mk<C>(C1, S1));
+    A[1][2] = mul_tw<SIGN>(A[1][2], mk<C>(H, H));
+    A[1][3] = mul_tw<SIGN>(A[1][3], mk<C>(S1, C1));
+    A[2][1] = mul_tw<SIGN>(A[2][1], mk<C>(H, H));
     A[2][2] = mul_si<SIGN>(A[2][2]);
-    A[2][3] = mul_tw<SIGN>(A[2][3], make_double2(-H, H));
-    A[3][1] = mul_tw<SIGN>(A[3][1], make_double2(S1, C1));
-    A[3][2] = mul_tw<SIGN>(A[3][2], make_double2(-H, H));
-    A[3][3] = mul_tw<SIGN>(A[3][3], make_double2(-C1, -S1));
+    A[2][3] = mul_tw<SIGN>(A[2][3], mk<C>(-H, H));
+    A[3][1] = mul_tw<SIGN>(A[3][1], mk<C>(S1, C1));
+    A[3][2] = mul_tw<SIGN>(A[3][2], mk<C>(-H, H));
+    A[3][3] = mul_tw<SIGN>(A[3][3], mk<C>(-C1, -S1));
 #pragma unroll
     for (int j1 = 0; j1 < 4; j1++) {
-      double2 c[4] = {A[0][j1], A[1][j1], A[2][j1], A[3][j1]};
+      C c[4] = {A[0][j1], A[1][j1], A[2][j1], A[3][j1]};
       Dft<4, SIGN>::run(c);
 #pragma unroll
       for (int j2 = 0; j2 < 4; j2++) v[j1 + 4 * j2] = c[j2];
@@ -182,15 +194,15 @@ template <int N, int T> struct LineMajor {   // one pad slot per 8 (per 4: x pas
 // registers, so no stage waits on a table read. Slot j of stage S holds tw[j * p * S] for the j that twiddles<R>
 // reads from the table; the other powers are formed from these exactly as twiddles<R> forms them.
 __host__ __device__ constexpr bool tw_from_table(int j) { return !NSL_TW_PRODUCTS || j < 4 || (j & 3) == 0; }
-template <int N, int T, int NT, class LY, int S, class L> struct TwPre {
-  __device__ __forceinline__ void load(const double2 *__restrict__) {}
+template <int N, int T, int NT, class LY, int S, class L, class C = double2> struct TwPre {
+  __device__ __forceinline__ void load(const C *__restrict__) {}
 };
-template <int N, int T, int NT, class LY, int S, int R0, int... Rs> struct TwPre<N, T, NT, LY, S, RList<R0, Rs...>> {
+template <int N, int T, int NT, class LY, int S, int R0, int... Rs, class C> struct TwPre<N, T, NT, LY, S, RList<R0, Rs...>, C> {
   static constexpr int NB = N / R0, TOT = NB * T, K = (TOT + NT - 1) / NT;
   static constexpr bool HAS = S * R0 != N;
-  double2 w[HAS ? K : 1][R0];
-  TwPre<N, T, NT, LY, S * R0, RList<Rs...>> next;
-  __device__ __forceinline__ void load(const double2 *__restrict__ tw) {
+  C w[HAS ? K : 1][R0];
+  TwPre<N, T, NT, LY, S * R0, RList<Rs...>, C> next;
+  __device__ __forceinline__ void load(const C *__restrict__ tw) {
     if constexpr (HAS) {
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -200,7 +212,7 @@ template <int N, int T, int NT, class LY, int S, int R0, int... Rs> struct TwPre
         const int base = (bb / S) * S;
 #pragma unroll
         for (int j = 1; j < R0; j++)
-          if (tw_from_table(j)) w[k][j] = (TOT % NT == 0 || idx < TOT) ? tw[j * base] : make_double2(1.0, 0.0);
+          if (tw_from_table(j)) w[k][j] = (TOT % NT == 0 || idx < TOT) ? tw[j * base] : mk<C>(1, 0);
       }
     }
     next.load(tw);
@@ -210,11 +222,11 @@ template <int N, int T, int NT, class LY, int S, int R0, int... Rs> struct TwPre
     else return next.template at<SS>();
   }
 };
-template <int N, int T, int NT, class LY, class SCH> using TwCache = TwPre<N, T, NT, LY, 1, SCH>;
+template <int N, int T, int NT, class LY, class SCH, class C = double2> using TwCache = TwPre<N, T, NT, LY, 1, SCH, C>;
 
 // The twiddles of slot k of a stage: from the table (TWS a pointer) or from a TwCache.
-template <int R, int S, class TWS>
-__device__ __forceinline__ void stage_twiddles(TWS &tws, int k, int base, double2 (&w)[R]) {
+template <int R, int S, class TWS, class C>
+__device__ __forceinline__ void stage_twiddles(TWS &tws, int k, int base, C (&w)[R]) {
   if constexpr (!std::is_class<typename std::remove_cv<TWS>::type>::value) {   // the table itself
     twiddles<R>(tws, base, w);
   } else {
@@ -247,10 +259,10 @@ template <bool LW, int NT = 64> __device__ __forceinline__ void stage_sync() {
 // One stage. S = product of the radices before it. Input either from `load(n, t)` (FIRST) or LDS; output to
 // `store(n, t, v)` (LAST) or LDS, at the layout LY's addresses.
 template <int N, int T, int NT, int R, int S, bool FIRST, bool LAST, int SIGN, bool SYNC_FIRST, bool LW, class LY,
-          class TWS, class LD, class ST>
-__device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &load, ST &store) {
+          class TWS, class LD, class ST, class C>
+__device__ __forceinline__ void stage(C *__restrict__ lds, TWS &tw, LD &load, ST &store) {
   constexpr int NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
-  double2 v[K][R];
+  C v[K][R];
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const int idx = threadIdx.x + k * NT;
@@ -277,7 +289,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
       const int p = bb / S, q = bb % S;
       Dft<R, SIGN>::run(v[k]);
       if (S * R != N) {
-        double2 w[R];
+        C w[R];
         stage_twiddles<R, S>(tw, k, p * S, w);
 #pragma unroll
         for (int j = 1; j < R; j++) v[k][j] = mul_tw<SIGN>(v[k][j], w[j]);
@@ -288,7 +300,7 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
         const int n = o0 + j * S;
         if constexpr (LAST) {
           // a store functor may also take the register slot (k, j): outputs can stay in registers
-          if constexpr (std::is_invocable<ST, int, int, double2, int, int>::value) store(n, t, v[k][j], k, j);
+          if constexpr (std::is_invocable<ST, int, int, C, int, int>::value) store(n, t, v[k][j], k, j);
           else store(n, t, v[k][j]);
         } else {
           lds[LY::at(n, t)] = v[k][j];
@@ -304,17 +316,17 @@ __device__ __forceinline__ void stage(double2 *__restrict__ lds, TWS &tw, LD &lo
 template <class L> struct LastR;
 template <int R0> struct LastR<RList<R0>> { static constexpr int R = R0; };
 template <int R0, int R1, int... Rs> struct LastR<RList<R0, R1, Rs...>> { static constexpr int R = LastR<RList<R1, Rs...>>::R; };
-template <int N, int T, int NT> struct OutRegs {
+template <int N, int T, int NT, class C = double2> struct OutRegs {
   static constexpr int R = LastR<typename Sched<N>::L>::R, S = N / R, TOT = S * T, K = (TOT + NT - 1) / NT;
-  double2 v[K][R];
+  C v[K][R];
   static __device__ __forceinline__ bool valid(int k) { return TOT % NT == 0 || (int)threadIdx.x + k * NT < TOT; }
   static __device__ __forceinline__ int line(int k) { return ((int)threadIdx.x + k * NT) % T; }
   static __device__ __forceinline__ int elem(int k, int j) { return ((int)threadIdx.x + k * NT) / T + j * S; }
 };
 
 template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, int S, bool FIRST, bool LW = false, class TWS,
-          class LD, class ST, int R0, int... Rs>
-__device__ __forceinline__ void run_stages(double2 *lds, TWS &tw, LD &load, ST &store, RList<R0, Rs...>) {
+          class LD, class ST, class C, int R0, int... Rs>
+__device__ __forceinline__ void run_stages(C *lds, TWS &tw, LD &load, ST &store, RList<R0, Rs...>) {
   constexpr bool LAST = sizeof...(Rs) == 0;
   stage<N, T, NT, R0, S, FIRST, LAST, SIGN, SYNC_FIRST, LW, LY>(lds, tw, load, store);
   if constexpr (!LAST)
@@ -323,92 +335,30 @@ __device__ __forceinline__ void run_stages(double2 *lds, TWS &tw, LD &load, ST &
 
 // Full transform of T lines of length N. tw: the N roots of unity (cos, sin)(2 pi k / N). SYNC_FIRST: the first
 // stage's loads read LDS that the stage then overwrites (in-place use), so a barrier separates them.
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleaved<N, T>, class TWS, class LD, class ST>
-__device__ __forceinline__ void fft(double2 *lds, TWS &tw, LD &load, ST &store) {
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY = Interleaved<N, T>, class TWS, class LD, class ST,
+          class C>
+__device__ __forceinline__ void fft(C *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, typename Sched<N>::L{});
 }
 // The same with an explicit radix schedule SCH (an RList whose product is N); tw is the table or a
 // TwCache<N, T, NT, LY, SCH>.
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
-__device__ __forceinline__ void fft_sched(double2 *lds, TWS &tw, LD &load, ST &store) {
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST, class C>
+__device__ __forceinline__ void fft_sched(C *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true>(lds, tw, load, store, SCH{});
 }
 // The same with LDS-only barriers (stage_sync<true, NT>), for kernels with loads into LDS in flight.
-template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST>
-__device__ __forceinline__ void fft_sched_lds(double2 *lds, TWS &tw, LD &load, ST &store) {
+template <int N, int T, int NT, int SIGN, bool SYNC_FIRST, class LY, class SCH, class TWS, class LD, class ST, class C>
+__device__ __forceinline__ void fft_sched_lds(C *lds, TWS &tw, LD &load, ST &store) {
   run_stages<N, T, NT, SIGN, SYNC_FIRST, LY, 1, true, true>(lds, tw, load, store, SCH{});
 }
 
-// ------------------------------------------------------------------------------------------------ split form
-// The first stage's inputs held in registers, so a persistent workgroup can issue the loads of its next tile before
-// it transforms the current one (the loads of tile i+1 are in flight during the LDS stages of tile i).
+// First radix of a schedule (and the rest of it).
 template <class L> struct FirstR;
 template <int R0, int... Rs> struct FirstR<RList<R0, Rs...>> {
   static constexpr int R = R0;
   using Rest = RList<Rs...>;
   static constexpr bool single = sizeof...(Rs) == 0;
 };
-
-template <int N, int T, int NT> struct Regs {
-  using F = FirstR<typename Sched<N>::L>;
-  static constexpr int R = F::R, NB = N / R, TOT = NB * T, K = (TOT + NT - 1) / NT;
-  double2 v[K][R];
-};
-
-template <int N, int T, int NT, class LD>
-__device__ __forceinline__ void load_first(Regs<N, T, NT> &r, LD &load) {
-  using RG = Regs<N, T, NT>;
-#pragma unroll
-  for (int k = 0; k < RG::K; k++) {
-    const int idx = threadIdx.x + k * NT;
-    if (RG::TOT % NT == 0 || idx < RG::TOT) {
-      const int t = idx % T, bb = idx / T;
-#pragma unroll
-      for (int j = 0; j < RG::R; j++) r.v[k][j] = load(bb + j * RG::NB, t);
-    }
-  }
-}
-
-template <int N, int T, int NT, int SIGN, class ST, int... Rs>
-__device__ __forceinline__ void rest_stages(double2 *lds, const double2 *tw, ST &store, RList<Rs...>) {
-  if constexpr (sizeof...(Rs) > 0) {
-    auto none = [](int, int) { return make_double2(0.0, 0.0); };
-    run_stages<N, T, NT, SIGN, false, Interleaved<N, T>, Regs<N, T, NT>::R, false>(lds, tw, none, store, RList<Rs...>{});
-  }
-}
-
-// Stage 1 from registers (no loads), then the remaining stages through LDS; outputs go to `store`.
-template <int N, int T, int NT, int SIGN, class ST>
-__device__ __forceinline__ void fft_regs(double2 *__restrict__ lds, const double2 *__restrict__ tw, Regs<N, T, NT> &r,
-                                         ST &store) {
-  using RG = Regs<N, T, NT>;
-  constexpr int R = RG::R, S = 1;
-  constexpr bool LAST = RG::F::single;
-#pragma unroll
-  for (int k = 0; k < RG::K; k++) {
-    const int idx = threadIdx.x + k * NT;
-    if (RG::TOT % NT == 0 || idx < RG::TOT) {
-      const int t = idx % T, bb = idx / T;
-      const int p = bb / S, q = bb % S;
-      Dft<R, SIGN>::run(r.v[k]);
-      if (S * R != N) {
-        double2 w[R];
-        twiddles<R>(tw, p * S, w);
-#pragma unroll
-        for (int j = 1; j < R; j++) r.v[k][j] = mul_tw<SIGN>(r.v[k][j], w[j]);
-      }
-      const int o0 = q + S * R * p;
-#pragma unroll
-      for (int j = 0; j < R; j++) {
-        const int n = o0 + j * S;
-        if (LAST) store(n, t, r.v[k][j]);
-        else lds[n * T + t] = r.v[k][j];
-      }
-    }
-  }
-  if (!LAST) __syncthreads();
-  rest_stages<N, T, NT, SIGN>(lds, tw, store, typename RG::F::Rest{});
-}
 
 }  // namespace fast
 }  // namespace nsl

@@ -105,49 +105,51 @@ template <int N, int T> struct ZinvFin {
 };
 template <int N, int T, class E = double2>
 __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(const E *__restrict__ U, E *__restrict__ X, const Geo,
-                                                      const double2 *__restrict__ tw, Ctl *__restrict__ ctl, size_t sb,
+                                                      const E *__restrict__ tw, Ctl *__restrict__ ctl, size_t sb,
                                                       const double *__restrict__ partials, int nfin, int fmode) {
+  using C = E;   // the transforms and the curl run in the storage precision
+  using F = Sc<C>;
   U = at_slot(U, sb); X = at_slot(X, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int first = nfin > 0 ? 1 : 0, ntiles = (int)gridDim.x - first;
   if ((int)blockIdx.x < first) {
     if constexpr (ZinvFin<N, T>::CH > 0)
       if (ctl->pend != 0)
-        fin_rhs<ZinvNT<N>::v, ZinvFin<N, T>::CH>(at_slot(partials, sb), nfin, fmode, nullptr, ctl, (double *)lds);
+        fin_rhs<ZinvNT<N>::v, ZinvFin<N, T>::CH>(at_slot(partials, sb), nfin, fmode, nullptr, ctl, (double *)lds_raw);
     return;
   }
   constexpr Geo g = trunc_geo<N>();
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
-  using OR = OutRegs<N, T, ZinvNT<N>::v>;
+  using OR = OutRegs<N, T, ZinvNT<N>::v, C>;
   const int bid = xcd_remap((int)blockIdx.x - first, ntiles);
   const int tile = bid % ntile, rest = bid / ntile, type = rest & 1, cy = rest >> 1, kx0 = tile * T;
-  const double ky = kwv(full_of(cy, g), N);
+  const F ky = (F)kwv(full_of(cy, g), N);
   const size_t ubase = (size_t)cy * g.hp + kx0;
   E *Xb = X + (size_t)cy * g.hp + kx0;
   OR A, B;
-  auto keepA = [&](int, int, double2 v, int k, int j) { A.v[k][j] = v; };
-  auto keepB = [&](int, int, double2 v, int k, int j) { B.v[k][j] = v; };
+  auto keepA = [&](int, int, C v, int k, int j) { A.v[k][j] = v; };
+  auto keepB = [&](int, int, C v, int k, int j) { B.v[k][j] = v; };
   // Below N = 512 the i kz factor multiplies the selected value (zero off the retained band) after the load's
   // branch, so no branch holds arithmetic on a loaded value and a transform's loads stay in flight together
   // (256^3: 118 -> 110 us). At N >= 512 that needs more than the 128-register cap and spills (993 -> 1046 us), so
   // the product stays on the load's path there.
   auto load_c = [&](int c, bool ikz) {
-    return [=](int n, int t, int, int j) -> double2 {
+    return [=](int n, int t, int, int j) -> C {
       const int cz = compact_of(n, g);
       if constexpr (N >= 512) {
-        if (cz < 0 || kx0 + t >= g.hx) return make_double2(0.0, 0.0);
-        const double2 u = ldc(U[(size_t)c * g.D + ubase + (size_t)cz * plane + t]);
+        if (cz < 0 || kx0 + t >= g.hx) return mk<C>(0, 0);
+        const C u = U[(size_t)c * g.D + ubase + (size_t)cz * plane + t];
         if (!ikz) return u;
-        const double kz = kwv(n, N);
-        return make_double2(-kz * u.y, kz * u.x);
+        const F kz = (F)kwv(n, N);
+        return mk<C>(-kz * u.y, kz * u.x);
       } else {
-        if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
-        const double2 u = (cz >= 0 && kx0 + t < g.hx) ? ldc(U[(size_t)c * g.D + ubase + (size_t)cz * plane + t])
-                                                      : make_double2(0.0, 0.0);
-        const double kz = kwv(n, N);
-        return ikz ? make_double2(-kz * u.y, kz * u.x) : u;
+        if (zero_slot<N>(j)) return mk<C>(0, 0);
+        const C u = (cz >= 0 && kx0 + t < g.hx) ? U[(size_t)c * g.D + ubase + (size_t)cz * plane + t] : mk<C>(0, 0);
+        const F kz = (F)kwv(n, N);
+        return ikz ? mk<C>(-kz * u.y, kz * u.x) : u;
       }
     };
   };
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
     for (int k = 0; k < OR::K; k++) {
       if (!OR::valid(k) || (!NSL_FULL_LINES && kx0 + OR::line(k) >= g.hx)) continue;
 #pragma unroll
-      for (int j = 0; j < OR::R; j++) stc(Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)], f(k, j));
+      for (int j = 0; j < OR::R; j++) Xb[((size_t)o * N + OR::elem(k, j)) * plane + OR::line(k)] = f(k, j);
     }
   };
   if (type == 0) {
@@ -168,9 +170,9 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
     fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, ly, keepB);
     out(1, [&](int k, int j) { return B.v[k][j]; });
     out(5, [&](int k, int j) {
-      const double kx = (double)(kx0 + OR::line(k));
-      const double pr = kx * B.v[k][j].x - ky * A.v[k][j].x, pi = kx * B.v[k][j].y - ky * A.v[k][j].y;
-      return make_double2(-pi, pr);
+      const F kx = (F)(kx0 + OR::line(k));
+      const F pr = kx * B.v[k][j].x - ky * A.v[k][j].x, pi = kx * B.v[k][j].y - ky * A.v[k][j].y;
+      return mk<C>(-pi, pr);
     });
   } else {
     auto lz = load_c(2, false);
@@ -179,13 +181,13 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
     auto ly = load_c(1, true);
     fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, ly, keepB);
     out(3, [&](int k, int j) {
-      return make_double2(-ky * A.v[k][j].y - B.v[k][j].x, ky * A.v[k][j].x - B.v[k][j].y);
+      return mk<C>(-ky * A.v[k][j].y - B.v[k][j].x, ky * A.v[k][j].x - B.v[k][j].y);
     });
     auto lx = load_c(0, true);
     fft<N, T, ZinvNT<N>::v, 1, false>(lds, tw, lx, keepB);
     out(4, [&](int k, int j) {
-      const double kx = (double)(kx0 + OR::line(k));
-      return make_double2(B.v[k][j].x + kx * A.v[k][j].y, B.v[k][j].y - kx * A.v[k][j].x);
+      const F kx = (F)(kx0 + OR::line(k));
+      return mk<C>(B.v[k][j].x + kx * A.v[k][j].y, B.v[k][j].y - kx * A.v[k][j].x);
     });
   }
 }
@@ -193,25 +195,27 @@ __global__ __launch_bounds__(ZinvNT<N>::v, ZinvMinWaves<N>::v) void kf_zinv_uw(c
 // P2: X1[o][z][cy][kx] -> X2[o][z][y][kx]. Grid (ntile*N, ncomp).
 template <int N, int T, class E = double2>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const E *__restrict__ X1, E *__restrict__ X2, const Geo,
-                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+                                                   const E *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb) {
+  using C = E;   // the transform runs in the storage precision
   X1 = at_slot(X1, sb); X2 = at_slot(X2, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   constexpr bool cut = Past<N, T, E>::cut;
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, o = blockIdx.y, kx0 = tile * T;
   const E *Ib = X1 + ((size_t)o * N + z) * g.by * g.hp + kx0;
   E *Ob = X2 + ((size_t)o * N + z) * N * g.hp + kx0;
-  auto load = [&](int n, int t, int, int j) -> double2 {
-    if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
+  auto load = [&](int n, int t, int, int j) -> C {
+    if (zero_slot<N>(j)) return mk<C>(0, 0);
     const int cy = compact_of(n, g);
-    return (cy >= 0 && kx0 + t < g.hx) ? ldc(Ib[cy * g.hp + t]) : make_double2(0.0, 0.0);
+    return (cy >= 0 && kx0 + t < g.hx) ? Ib[cy * g.hp + t] : mk<C>(0, 0);
   };
   // every column of the tile is stored, padding kx >= hx included (its inputs are zero, so it stores zeros): whole
   // 128-byte lines, never a partial line
-  auto store = [&](int n, int t, double2 v) {
-    if ((NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) stc(Ob[n * g.hp + t], v);
+  auto store = [&](int n, int t, C v) {
+    if ((NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) Ob[n * g.hp + t] = v;
   };
   fft<N, T, PassNT<N>::v, 1, false>(lds, tw, load, store);
 }
@@ -219,8 +223,9 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yinv(const E *__restrict__ X1
 // P4: X3[c][z][y][kx] -> X4[c][z][cy][kx] (retained ky). Grid (ntile*N, ncomp).
 template <int N, int T, class E = double2>
 __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const E *__restrict__ X3, E *__restrict__ X4, const Geo,
-                                                   const double2 *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb,
+                                                   const E *__restrict__ tw, const Ctl *__restrict__ ctl, size_t sb,
                                                    const double *__restrict__ pmax, int nblk, int nred) {
+  using C = E;   // the transform runs in the storage precision
   X3 = at_slot(X3, sb); X4 = at_slot(X4, sb); ctl = at_slot(ctl, sb);
   if (ctl != nullptr && ctl->active == 0) return;
   // The x pass's per-workgroup maxima (max |omega|^2, max |u|^2), when given: the first nred workgroups of component 0
@@ -238,19 +243,20 @@ __global__ __launch_bounds__(PassNT<N>::v) void kf_yfwd(const E *__restrict__ X3
     Ctl *cw = const_cast<Ctl *>(ctl);
     block_atomic_max2<NT>(a, b, &cw->om2, &cw->u2, mscr);
   }
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   constexpr bool cut = Past<N, T, E>::cut;
   const int ntile = (g.hx + T - 1) / T;
   const int tile = blockIdx.x % ntile, z = blockIdx.x / ntile, c = blockIdx.y, kx0 = tile * T;
   const E *Ib = X3 + ((size_t)c * N + z) * N * g.hp + kx0;
   E *Ob = X4 + ((size_t)c * N + z) * g.by * g.hp + kx0;
-  auto load = [&](int n, int t) -> double2 {
-    return kx0 + t < g.hx ? ldc(Ib[n * g.hp + t]) : make_double2(0.0, 0.0);
+  auto load = [&](int n, int t) -> C {
+    return kx0 + t < g.hx ? Ib[n * g.hp + t] : mk<C>(0, 0);
   };
-  auto store = [&](int n, int t, double2 v) {   // whole tiles, as in the y-inverse
+  auto store = [&](int n, int t, C v) {   // whole tiles, as in the y-inverse
     const int cy = compact_of(n, g);
-    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) stc(Ob[cy * g.hp + t], v);
+    if (cy >= 0 && (NSL_FULL_LINES || kx0 + t < g.hx) && (!cut || kx0 + t < g.hp)) Ob[cy * g.hp + t] = v;
   };
   fft<N, T, PassNT<N>::v, -1, false>(lds, tw, load, store);
 }
@@ -346,15 +352,17 @@ template <int N> struct ZuSlots {
 template <int N, int STAGE, int V = 0, class E = double2>
 __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(const E *__restrict__ X4, const E *U,
                                                             E *S, E *__restrict__ ACC, E *TS,
-                                                            const Geo, const double2 *__restrict__ tw, double inv,
+                                                            const Geo, const E *__restrict__ tw, double inv,
                                                             double *__restrict__ partials, Ctl *__restrict__ ctl,
                                                             size_t sb) {
   using C = ZuCfg<N>;
+  using CT = E;   // the z-forward runs in the storage precision; the update below in FP64
   constexpr int T = C::T, R = C::R, NB = C::NB, NT = C::NT;
   X4 = at_slot(X4, sb); U = at_slot(U, sb); S = at_slot(S, sb); ACC = at_slot(ACC, sb); TS = at_slot(TS, sb);
   partials = at_slot(partials, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  CT *lds = reinterpret_cast<CT *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   static_assert(g.hp % T == 0, "whole tiles cover the row pitch");
   constexpr int ntile = (g.hx + T - 1) / T, plane = g.by * g.hp;
@@ -368,10 +376,10 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
   double2 Nl[3][R];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    auto load = [&](int n, int t) -> double2 {
-      return kx0 + t < g.hx ? ldc(Ib[((size_t)c * N + n) * plane + t]) : make_double2(0.0, 0.0);
+    auto load = [&](int n, int t) -> CT {
+      return kx0 + t < g.hx ? Ib[((size_t)c * N + n) * plane + t] : mk<CT>(0, 0);
     };
-    auto keep = [&](int, int, double2 v, int, int j) { Nl[c][j] = make_double2(v.x * inv, v.y * inv); };
+    auto keep = [&](int, int, CT v, int, int j) { Nl[c][j] = make_double2((double)v.x * inv, (double)v.y * inv); };
     fft_sched<N, T, NT, -1, false, Interleaved<N, T>, typename ZuSchedV<N, V>::L>(lds, tw, load, keep);
   }
   const double nu = ctl->nu;
@@ -406,7 +414,7 @@ __global__ __launch_bounds__(ZuCfg<N>::NT, ZuCfg<N>::MINW) void kf_zfwd_upd(cons
       upd_apply<STAGE>(in[q], m, g.D, kx, ky, kwv(n, N), h, Nl[0][j], Nl[1][j], Nl[2][j], S, ACC, TS, nu, dt, acc);
     }
   }
-  block_sum<8, NT>(acc, reinterpret_cast<double *>(lds), partials + (size_t)bid * 8);
+  block_sum<8, NT>(acc, reinterpret_cast<double *>(lds_raw), partials + (size_t)bid * 8);
   // the booking of these partials (fin_rhs) is left to the next launch that reads them: the next rhs's z-inverse,
   // the next step's k_begin_step, or a k_finalize at the end of the steps enqueued
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->pend = 1;
@@ -421,11 +429,14 @@ using XnlLayout = typename std::conditional<NSL_XNL_LINEMAJOR != 0, LineMajor<N,
 
 // P3: in place on X2 (6 fields in, 3 out), LP row pairs per workgroup. Same packing and product as k_xnl.
 template <int N, int LP, class E = double2>
-__global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+__global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, const E *__restrict__ tw,
                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  using C = E;   // the transforms and the product run in the storage precision
+  using F = Sc<C>;
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   constexpr int T = 6 * LP, T2 = 3 * LP, half = N / 2, npairs = N * half;
   using L1 = XnlLayout<N, T>;
@@ -438,20 +449,20 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, 
   constexpr int ngroups = (npairs + LP - 1) / LP;
   // one group of LP row pairs per workgroup (a persistent loop with prefetch needs 256+ VGPRs on gfx950 and
   // measured slower)
-  double2 A[KL], B[KL];
+  C A[KL], B[KL];
   auto load_group = [&](int grp) {
 #pragma unroll
     for (int i = 0; i < KL; i++) {
       const int e = threadIdx.x + i * NSL_NT;
-      A[i] = B[i] = make_double2(0.0, 0.0);
+      A[i] = B[i] = mk<C>(0.0, 0.0);
       if (ITEMS % NSL_NT == 0 || e < ITEMS) {
         const int t = e / hx, k = e - t * hx, p = t / 6, o = t - p * 6;
         const int pr = grp * LP + p;
         if (pr < npairs) {
           const int z = pr / half, y = 2 * (pr - z * half);
           const size_t base = o * cs + ((size_t)z * N + y) * HP;
-          A[i] = ldc(X2[base + k]);
-          B[i] = ldc(X2[base + HP + k]);
+          A[i] = (X2[base + k]);
+          B[i] = (X2[base + HP + k]);
         }
       }
     }
@@ -465,37 +476,37 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, 
     const int e = threadIdx.x + i * NSL_NT;
     if (ITEMS % NSL_NT == 0 || e < ITEMS) {
       const int t = e / hx, k = e - t * hx;
-      lds[L1::at(k, t)] = make_double2(A[i].x - B[i].y, A[i].y + B[i].x);
-      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = make_double2(A[i].x + B[i].y, B[i].x - A[i].y);
+      lds[L1::at(k, t)] = mk<C>(A[i].x - B[i].y, A[i].y + B[i].x);
+      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = mk<C>(A[i].x + B[i].y, B[i].x - A[i].y);
     }
   }
   __syncthreads();
   // modes hx .. N-hx of every line are zero: never written, never read
-  auto ldl = [&](int n, int t) -> double2 {
-    return (n >= hx && n <= N - hx) ? make_double2(0.0, 0.0) : lds[L1::at(n, t)];
+  auto ldl = [&](int n, int t) -> C {
+    return (n >= hx && n <= N - hx) ? mk<C>(0.0, 0.0) : lds[L1::at(n, t)];
   };
-  auto stl = [&](int n, int t, double2 v) { lds[L1::at(n, t)] = v; };
+  auto stl = [&](int n, int t, C v) { lds[L1::at(n, t)] = v; };
   fft<N, T, NSL_NT, 1, true, L1>(lds, tw, ldl, stl);
   __syncthreads();
   constexpr int KI = (N * LP + NSL_NT - 1) / NSL_NT;
-  double nv[KI][6];
+  F nv[KI][6];
 #pragma unroll
   for (int k2 = 0; k2 < KI; k2++) {
     const int it = threadIdx.x + k2 * NSL_NT;
     if ((N * LP) % NSL_NT == 0 || it < N * LP) {
       const int x = it % N, p = it / N;
-      const double2 Za = lds[L1::at(x, p * 6)], Zb = lds[L1::at(x, p * 6 + 1)], Zc = lds[L1::at(x, p * 6 + 2)];
-      const double2 Zp = lds[L1::at(x, p * 6 + 3)], Zq = lds[L1::at(x, p * 6 + 4)], Zr = lds[L1::at(x, p * 6 + 5)];
+      const C Za = lds[L1::at(x, p * 6)], Zb = lds[L1::at(x, p * 6 + 1)], Zc = lds[L1::at(x, p * 6 + 2)];
+      const C Zp = lds[L1::at(x, p * 6 + 3)], Zq = lds[L1::at(x, p * 6 + 4)], Zr = lds[L1::at(x, p * 6 + 5)];
       {
-        const double a = Za.x, b = Zb.x, c = Zc.x, pp = Zp.x, q = Zq.x, r = Zr.x;
-        const double o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
-        om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+        const F a = Za.x, b = Zb.x, c = Zc.x, pp = Zp.x, q = Zq.x, r = Zr.x;
+        const F o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
+        om2 = fmax(om2, (double)o2); u2 = fmax(u2, (double)uu);
         nv[k2][0] = b * r - c * q; nv[k2][1] = c * pp - a * r; nv[k2][2] = a * q - b * pp;
       }
       {
-        const double a = Za.y, b = Zb.y, c = Zc.y, pp = Zp.y, q = Zq.y, r = Zr.y;
-        const double o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
-        om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+        const F a = Za.y, b = Zb.y, c = Zc.y, pp = Zp.y, q = Zq.y, r = Zr.y;
+        const F o2 = pp * pp + q * q + r * r, uu = a * a + b * b + c * c;
+        om2 = fmax(om2, (double)o2); u2 = fmax(u2, (double)uu);
         nv[k2][3] = b * r - c * q; nv[k2][4] = c * pp - a * r; nv[k2][5] = a * q - b * pp;
       }
     }
@@ -506,15 +517,15 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, 
     const int it = threadIdx.x + k2 * NSL_NT;
     if ((N * LP) % NSL_NT == 0 || it < N * LP) {
       const int x = it % N, p = it / N;
-      lds[L2::at(x, p * 3)] = make_double2(nv[k2][0], nv[k2][3]);
-      lds[L2::at(x, p * 3 + 1)] = make_double2(nv[k2][1], nv[k2][4]);
-      lds[L2::at(x, p * 3 + 2)] = make_double2(nv[k2][2], nv[k2][5]);
+      lds[L2::at(x, p * 3)] = mk<C>(nv[k2][0], nv[k2][3]);
+      lds[L2::at(x, p * 3 + 1)] = mk<C>(nv[k2][1], nv[k2][4]);
+      lds[L2::at(x, p * 3 + 2)] = mk<C>(nv[k2][2], nv[k2][5]);
     }
   }
   __syncthreads();
-  auto ldl2 = [&](int n, int t) -> double2 { return lds[L2::at(n, t)]; };
+  auto ldl2 = [&](int n, int t) -> C { return lds[L2::at(n, t)]; };
   // only modes k < hx and their mirrors N-k are unpacked
-  auto stl2 = [&](int n, int t, double2 v) {
+  auto stl2 = [&](int n, int t, C v) {
     if (n < hx || n > N - hx) lds[L2::at(n, t)] = v;
   };
   fft<N, T2, NSL_NT, -1, true, L2>(lds, tw, ldl2, stl2);
@@ -529,30 +540,33 @@ __global__ __launch_bounds__(NSL_NT) void kf_xnl(E *__restrict__ X2, const Geo, 
     const int pr = grp * LP + p;
     if (pr >= npairs) continue;
     const int z = pr / half, y = 2 * (pr - z * half);
-    double2 X = make_double2(0.0, 0.0), Y = X;
+    C X = mk<C>(0.0, 0.0), Y = X;
     if (k < hx) {
-      const double2 Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
-      const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-      X = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-      Y = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+      const C Zk = lds[L2::at(k, t)], Zc = lds[L2::at((N - k) % N, t)];
+      const F zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+      X = mk<C>(0.5 * (zr + cr), 0.5 * (zi + ci));
+      Y = mk<C>(0.5 * (zi - ci), -0.5 * (zr - cr));
     }
     const size_t base = c * cs + ((size_t)z * N + y) * HP;
-    stc(X2[base + k], X);
-    stc(X2[base + HP + k], Y);
+    X2[base + k] = X;
+    X2[base + HP + k] = Y;
   }
   }
-  block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds);
+  block_max2_store(om2, u2, pmax + 2 * (size_t)blockIdx.x, (double *)lds_raw);
 }
 
 // P3 with RP rows per workgroup of NT threads (half the LDS of the row-pair form for RP = 1). Each row's six
 // half-spectra ride as three complex lines, (u_x + i u_y), (u_z + i om_x), (om_y + i om_z); after the inverse the
 // product u x omega is formed per point, and (n_x + i n_y), (n_z + i 0) go through the forward transform.
 template <int N, int RP, int NT, class E = double2>
-__global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+__global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo, const E *__restrict__ tw,
                                                    double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  using C = E;   // the transforms and the product run in the storage precision
+  using F = Sc<C>;
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   constexpr int T = 3 * RP, T2 = 2 * RP, nrows = N * N;
   using L1 = Interleaved<N, T>;
@@ -561,17 +575,17 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo,
   constexpr int ITEMS = T * hx, KL = (ITEMS + NT - 1) / NT;
   constexpr size_t cs = (size_t)N * N * HP;
   const int row0 = blockIdx.x * RP;
-  double2 A[KL], B[KL];
+  C A[KL], B[KL];
 #pragma unroll
   for (int i = 0; i < KL; i++) {
     const int e = threadIdx.x + i * NT;
-    A[i] = B[i] = make_double2(0.0, 0.0);
+    A[i] = B[i] = mk<C>(0.0, 0.0);
     if (ITEMS % NT == 0 || e < ITEMS) {
       const int t = e / hx, k = e - t * hx, r = t / 3, l = t - r * 3;   // line l of row r: fields 2l, 2l+1
       if (row0 + r < nrows) {
         const size_t base = (size_t)(row0 + r) * HP + k;
-        A[i] = ldc(X2[2 * l * cs + base]);
-        B[i] = ldc(X2[(2 * l + 1) * cs + base]);
+        A[i] = (X2[2 * l * cs + base]);
+        B[i] = (X2[(2 * l + 1) * cs + base]);
       }
     }
   }
@@ -580,29 +594,29 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo,
     const int e = threadIdx.x + i * NT;
     if (ITEMS % NT == 0 || e < ITEMS) {
       const int t = e / hx, k = e - t * hx;
-      lds[L1::at(k, t)] = make_double2(A[i].x - B[i].y, A[i].y + B[i].x);
-      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = make_double2(A[i].x + B[i].y, B[i].x - A[i].y);
+      lds[L1::at(k, t)] = mk<C>(A[i].x - B[i].y, A[i].y + B[i].x);
+      if (k > 0 && 2 * k < N) lds[L1::at(N - k, t)] = mk<C>(A[i].x + B[i].y, B[i].x - A[i].y);
     }
   }
   __syncthreads();
-  auto ldl = [&](int n, int t) -> double2 {
-    return (n >= hx && n <= N - hx) ? make_double2(0.0, 0.0) : lds[L1::at(n, t)];
+  auto ldl = [&](int n, int t) -> C {
+    return (n >= hx && n <= N - hx) ? mk<C>(0.0, 0.0) : lds[L1::at(n, t)];
   };
-  auto stl = [&](int n, int t, double2 v) { lds[L1::at(n, t)] = v; };
+  auto stl = [&](int n, int t, C v) { lds[L1::at(n, t)] = v; };
   fft<N, T, NT, 1, true, L1>(lds, tw, ldl, stl);
   __syncthreads();
   constexpr int KI = (N * RP + NT - 1) / NT;
-  double nv[KI][3];
+  F nv[KI][3];
   double om2 = 0.0, u2 = 0.0;
 #pragma unroll
   for (int k2 = 0; k2 < KI; k2++) {
     const int it = threadIdx.x + k2 * NT;
     if ((N * RP) % NT == 0 || it < N * RP) {
       const int x = it % N, r = it / N;
-      const double2 Z0 = lds[L1::at(x, 3 * r)], Z1 = lds[L1::at(x, 3 * r + 1)], Z2 = lds[L1::at(x, 3 * r + 2)];
-      const double a = Z0.x, b = Z0.y, c = Z1.x, pp = Z1.y, q = Z2.x, rr = Z2.y;   // u = (a, b, c), omega = (pp, q, rr)
-      const double o2 = pp * pp + q * q + rr * rr, uu = a * a + b * b + c * c;
-      om2 = fmax(om2, o2); u2 = fmax(u2, uu);
+      const C Z0 = lds[L1::at(x, 3 * r)], Z1 = lds[L1::at(x, 3 * r + 1)], Z2 = lds[L1::at(x, 3 * r + 2)];
+      const F a = Z0.x, b = Z0.y, c = Z1.x, pp = Z1.y, q = Z2.x, rr = Z2.y;   // u = (a, b, c), omega = (pp, q, rr)
+      const F o2 = pp * pp + q * q + rr * rr, uu = a * a + b * b + c * c;
+      om2 = fmax(om2, (double)o2); u2 = fmax(u2, (double)uu);
       nv[k2][0] = b * rr - c * q; nv[k2][1] = c * pp - a * rr; nv[k2][2] = a * q - b * pp;
     }
   }
@@ -612,13 +626,13 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo,
     const int it = threadIdx.x + k2 * NT;
     if ((N * RP) % NT == 0 || it < N * RP) {
       const int x = it % N, r = it / N;
-      lds[L2::at(x, 2 * r)] = make_double2(nv[k2][0], nv[k2][1]);
-      lds[L2::at(x, 2 * r + 1)] = make_double2(nv[k2][2], 0.0);
+      lds[L2::at(x, 2 * r)] = mk<C>(nv[k2][0], nv[k2][1]);
+      lds[L2::at(x, 2 * r + 1)] = mk<C>(nv[k2][2], 0.0);
     }
   }
   __syncthreads();
-  auto ldl2 = [&](int n, int t) -> double2 { return lds[L2::at(n, t)]; };
-  auto stl2 = [&](int n, int t, double2 v) {
+  auto ldl2 = [&](int n, int t) -> C { return lds[L2::at(n, t)]; };
+  auto stl2 = [&](int n, int t, C v) {
     if (n < hx || n > N - hx) lds[L2::at(n, t)] = v;
   };
   fft<N, T2, NT, -1, true, L2>(lds, tw, ldl2, stl2);
@@ -633,27 +647,27 @@ __global__ __launch_bounds__(NT) void kf_xnl_rows(E *__restrict__ X2, const Geo,
     if (row0 + r >= nrows) continue;
     const size_t base = (size_t)(row0 + r) * HP + k;
     const int kk = (N - k) % N;
-    double2 o0 = make_double2(0.0, 0.0), o1 = o0, o2 = o0;
+    C o0 = mk<C>(0.0, 0.0), o1 = o0, o2 = o0;
     if (k < hx) {
       {  // n_x + i n_y
-        const double2 Zk = lds[L2::at(k, 2 * r)], Zc = lds[L2::at(kk, 2 * r)];
-        const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-        o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-        o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+        const C Zk = lds[L2::at(k, 2 * r)], Zc = lds[L2::at(kk, 2 * r)];
+        const F zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+        o0 = mk<C>(0.5 * (zr + cr), 0.5 * (zi + ci));
+        o1 = mk<C>(0.5 * (zi - ci), -0.5 * (zr - cr));
       }
       {  // n_z (imaginary part zero)
-        const double2 Zk = lds[L2::at(k, 2 * r + 1)], Zc = lds[L2::at(kk, 2 * r + 1)];
-        o2 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+        const C Zk = lds[L2::at(k, 2 * r + 1)], Zc = lds[L2::at(kk, 2 * r + 1)];
+        o2 = mk<C>(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
       }
     }
-    stc(X2[base], o0);
-    stc(X2[cs + base], o1);
-    stc(X2[2 * cs + base], o2);
+    X2[base] = o0;
+    X2[cs + base] = o1;
+    X2[2 * cs + base] = o2;
   }
   // per-workgroup maxima
   om2 = wave_max(om2); u2 = wave_max(u2);
   __syncthreads();
-  double *scratch = (double *)lds;
+  double *scratch = (double *)lds_raw;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) { scratch[2 * wid] = om2; scratch[2 * wid + 1] = u2; }
   __syncthreads();
@@ -686,15 +700,18 @@ template <> struct XnlWave<384> { using L = RList<8, 6, 8>; };
 template <> struct XnlWave<512> { using L = RList<8, 8, 8>; };
 template <> struct XnlWave<256> { using L = RList<8, 4, 8>; };
 template <int N, class E = double2>
-__global__ __launch_bounds__(64) void kf_xnl_wave(E *__restrict__ X2, const Geo, const double2 *__restrict__ tw,
+__global__ __launch_bounds__(64) void kf_xnl_wave(E *__restrict__ X2, const Geo, const E *__restrict__ tw,
                                                   double *__restrict__ pmax, Ctl *__restrict__ ctl, size_t sb) {
+  using C = E;   // the transforms and the product run in the storage precision
+  using F = Sc<C>;
   constexpr int R = 8, NB = N / R, RPW = 64 / NB;   // butterflies per line and stage ends, rows per wave
   static_assert(NB * RPW == 64, "first and last stage must cover a wave");
   using SCH = typename XnlWave<N>::L;
   static_assert(FirstR<SCH>::R == R && LastR<SCH>::R == R, "schedule starts and ends in radix 8");
   X2 = at_slot(X2, sb); pmax = at_slot(pmax, sb); ctl = at_slot(ctl, sb);
   if (ctl->active == 0) return;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
   constexpr size_t cs = (size_t)N * N * HP;
@@ -702,41 +719,41 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(E *__restrict__ X2, const Geo,
   const int lane = threadIdx.x;
   const int row0 = blockIdx.x * RPW;
   // the twiddles of this lane's butterflies are the same in all five transforms: read once
-  TwCache<N, RPW, 64, LY, SCH> twc;
+  TwCache<N, RPW, 64, LY, SCH, C> twc;
   twc.load(tw);
-  double2 U[3][R];
+  C U[3][R];
 #pragma unroll
   for (int l = 0; l < 3; l++) {
     // line l of row row0 + t: mode n < hx from the half-spectrum, n > N - hx from its mirror, zero between. Every
     // load is unconditional (the zero band reads index 0 and discards it), so a line's loads issue back to back and
     // are waited on once; slots j whose range j*NB .. j*NB+NB-1 lies in the zero band load nothing.
-    auto ld = [&](int n, int t, int, int j) -> double2 {
-      if (j * NB >= hx && j * NB + NB - 1 <= N - hx) return make_double2(0.0, 0.0);
+    auto ld = [&](int n, int t, int, int j) -> C {
+      if (j * NB >= hx && j * NB + NB - 1 <= N - hx) return mk<C>(0.0, 0.0);
       const size_t base = (size_t)(row0 + t) * HP;
       const E *Fa = X2 + 2 * l * cs + base, *Fb = X2 + (2 * l + 1) * cs + base;
       const bool lo = n < hx, hi = n > N - hx;
       const int m = lo ? n : (hi ? N - n : 0);
-      const double2 A = ldc(Fa[m]), B = ldc(Fb[m]);
-      const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
-      return lo || hi ? d : make_double2(0.0, 0.0);
+      const C A = Fa[m], B = Fb[m];
+      const C d = lo ? mk<C>(A.x - B.y, A.y + B.x) : mk<C>(A.x + B.y, B.x - A.y);
+      return lo || hi ? d : mk<C>(0.0, 0.0);
     };
-    auto keep = [&](int, int, double2 v, int, int j) { U[l][j] = v; };
+    auto keep = [&](int, int, C v, int, int j) { U[l][j] = v; };
     fft_sched<N, RPW, 64, 1, false, LY, SCH>(lds, twc, ld, keep);
   }
   // u x omega at x = bb + NB j of row row0 + lane / NB: the inverse's outputs of this lane
   double om2 = 0.0, u2 = 0.0;
-  double2 P[2][R];
+  C P[2][R];
 #pragma unroll
   for (int j = 0; j < R; j++) {
-    const double a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
-    om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c);
-    P[0][j] = make_double2(b * rr - c * q, c * pp - a * rr);
-    P[1][j] = make_double2(a * q - b * pp, 0.0);
+    const F a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
+    om2 = fmax(om2, (double)(pp * pp + q * q + rr * rr)); u2 = fmax(u2, (double)(a * a + b * b + c * c));
+    P[0][j] = mk<C>(b * rr - c * q, c * pp - a * rr);
+    P[1][j] = mk<C>(a * q - b * pp, 0.0);
   }
 #pragma unroll
   for (int l = 0; l < 2; l++) {
-    auto ld2 = [&](int, int, int, int j) -> double2 { return P[l][j]; };
-    auto st2 = [&](int n, int t, double2 v) {   // only modes k < hx and their mirrors N-k are unpacked
+    auto ld2 = [&](int, int, int, int j) -> C { return P[l][j]; };
+    auto st2 = [&](int n, int t, C v) {   // only modes k < hx and their mirrors N-k are unpacked
       if (n < hx || n > N - hx) lds[LY::at(n, t)] = v;
     };
     fft_sched<N, RPW, 64, -1, false, LY, SCH>(lds, twc, ld2, st2);
@@ -749,22 +766,22 @@ __global__ __launch_bounds__(64) void kf_xnl_wave(E *__restrict__ X2, const Geo,
       if (e >= RPW * HP) continue;
       const int t = e / HP, k = e - t * HP;
       const size_t base = (size_t)(row0 + t) * HP;
-      double2 o0 = make_double2(0.0, 0.0), o1 = o0;
+      C o0 = mk<C>(0.0, 0.0), o1 = o0;
       if (k < hx) {
-        const double2 Zk = lds[LY::at(k, t)], Zc = lds[LY::at((N - k) % N, t)];
+        const C Zk = lds[LY::at(k, t)], Zc = lds[LY::at((N - k) % N, t)];
         if (l == 0) {   // n_x + i n_y
-          const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
-          o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
-          o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+          const F zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+          o0 = mk<C>(0.5 * (zr + cr), 0.5 * (zi + ci));
+          o1 = mk<C>(0.5 * (zi - ci), -0.5 * (zr - cr));
         } else {        // n_z (imaginary part zero)
-          o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+          o0 = mk<C>(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
         }
       }
       if (l == 0) {
-        stc(X2[base + k], o0);
-        stc(X2[cs + base + k], o1);
+        X2[base + k] = o0;
+        X2[cs + base + k] = o1;
       } else {
-        stc(X2[2 * cs + base + k], o0);
+        X2[2 * cs + base + k] = o0;
       }
     }
     __syncthreads();   // the next line's first stage overwrites LDS

@@ -1,6 +1,8 @@
-// Passes of an FP32 solve: the spectral arrays in complex64 (float2) in HBM, the arithmetic in registers in FP64.
-// The same size-specialised rhs kernels (nsl_fastk.hpp) instantiated with storage E = float2, with the z-forward
-// fused into the update and one x-pass form per N. The y passes take 16-line tiles from 96^3 to 384^3: a row of 16
+// Passes of an FP32 solve: the spectral arrays in complex64 (float2) in HBM, the transforms, the curl and the product
+// u x omega in single precision (float2 registers and LDS, float twiddles x.twN32), the update (projection, viscous
+// term, RK4) and every sum in FP64 registers. The same size-specialised rhs kernels (nsl_fastk.hpp) instantiated with
+// storage E = float2, which is also their compute type, with the z-forward fused into the update and one x-pass
+// form per N. The y passes take 16-line tiles from 96^3 to 384^3: a row of 16
 // complex64 is one 128-byte line, where 8 lines read half lines (at 256^3, 8 lines fetched as many bytes as FP64:
 // y-forward 137 us, 16 lines 91 us; y-inverse 229 and 198 us). The z-inverse and the fused update keep the FP64
 // tiles: with 16 lines they ran slower (256^3: 171 against 112 us, one wave per SIMD at 256 registers; the update
@@ -8,10 +10,9 @@
 // so every launch has gridDim.z = 1 and sb = 0.
 //
 // The reference's FP32 mode (nslab_gpu.py:56-65) holds every array in complex64/float32 and transforms in single
-// precision. Here every array that lives in HBM between passes is complex64, so the state, the RK4 registers and
-// each pass's output carry FP32 rounding as there; the butterflies, the product u x omega and the update arithmetic
-// run in FP64 registers, which costs nothing on a path bound by HBM and gives a little less roundoff than an
-// all-single pipeline.
+// precision. Here too: every array that lives in HBM between passes is complex64, and the transforms and the
+// product run in single precision, so the state, the RK4 registers and each pass's output carry FP32 rounding as
+// there. The update and the sums stay FP64 (a little less roundoff than an all-single pipeline, at no cost).
 #pragma once
 
 // Sizes with an FP32 path: those with the fused z-forward and update.
@@ -44,17 +45,17 @@ template <int N> struct P32 {
     static_assert(fast::ZinvFin<N, TZ>::CH > 0, "the z-inverse's LDS holds the booking's scratch");
     allow_lds(fast::kf_zinv_uw<N, TZ, float2>, lds);
     hipLaunchKernelGGL((fast::kf_zinv_uw<N, TZ, float2>), dim3((unsigned)(ntile_z * 2 * g.by + (fin.nfin > 0 ? 1 : 0))),
-                       dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, x.twN, ctl, (size_t)0, fin.partials, fin.nfin,
-                       fin.fmode);
+                       dim3(fast::ZinvNT<N>::v), lds, x.st, U, X1, x.g, (const float2 *)x.twN32, ctl, (size_t)0,
+                       fin.partials, fin.nfin, fin.fmode);
     LAUNCH_CHECK();
     return NSL_OK;
   }
   // P2: six fields along y
   static int yinv(const Xform &x, const float2 *X1, float2 *X2, const Ctl *ctl) {
-    const int lds = N * T * (int)sizeof(double2);
+    const int lds = N * T * (int)sizeof(float2);
     allow_lds(fast::kf_yinv<N, T, float2>, lds);
     hipLaunchKernelGGL((fast::kf_yinv<N, T, float2>), dim3((unsigned)(ntile * N), 6u), dim3(fast::PassNT<N>::v), lds, x.st,
-                       X1, X2, x.g, x.twN, ctl, (size_t)0);
+                       X1, X2, x.g, (const float2 *)x.twN32, ctl, (size_t)0);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -64,22 +65,22 @@ template <int N> struct P32 {
     static_assert(grid > 0 && grid <= (long long)N * N, "x-pass maxima fit their buffer");
     if constexpr (xp32_form(N) == NSL_XP_WAVE) {
       constexpr int RPW = 64 / (N / 8);
-      const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(double2);
-      hipLaunchKernelGGL((fast::kf_xnl_wave<N, float2>), dim3((unsigned)grid), dim3(64), lds, x.st, X2, x.g, x.twN, pmax,
-                         ctl, (size_t)0);
+      const int lds = fast::LineMajor<N, RPW>::SIZE * (int)sizeof(float2);
+      hipLaunchKernelGGL((fast::kf_xnl_wave<N, float2>), dim3((unsigned)grid), dim3(64), lds, x.st, X2, x.g,
+                         (const float2 *)x.twN32, pmax, ctl, (size_t)0);
     } else if constexpr (xp32_form(N) == NSL_XP_ROWS) {
       constexpr int RP = 1, NT = 192;
-      const int lds = N * 3 * RP * (int)sizeof(double2);
+      const int lds = N * 3 * RP * (int)sizeof(float2);
       allow_lds(fast::kf_xnl_rows<N, RP, NT, float2>, lds);
-      hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT, float2>), dim3((unsigned)grid), dim3(NT), lds, x.st, X2, x.g, x.twN,
-                         pmax, ctl, (size_t)0);
+      hipLaunchKernelGGL((fast::kf_xnl_rows<N, RP, NT, float2>), dim3((unsigned)grid), dim3(NT), lds, x.st, X2, x.g,
+                         (const float2 *)x.twN32, pmax, ctl, (size_t)0);
     } else {
       constexpr int LP = fast_LP(N);
       static_assert(xp_pair_lp_ok(N, LP), "row pairs fit LDS");
-      const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(double2);
+      const int lds = fast::XnlLayout<N, 6 * LP>::SIZE * (int)sizeof(float2);
       allow_lds(fast::kf_xnl<N, LP, float2>, lds);
-      hipLaunchKernelGGL((fast::kf_xnl<N, LP, float2>), dim3((unsigned)grid), dim3(NSL_NT), lds, x.st, X2, x.g, x.twN, pmax,
-                         ctl, (size_t)0);
+      hipLaunchKernelGGL((fast::kf_xnl<N, LP, float2>), dim3((unsigned)grid), dim3(NSL_NT), lds, x.st, X2, x.g,
+                         (const float2 *)x.twN32, pmax, ctl, (size_t)0);
     }
     LAUNCH_CHECK();
     *nred = (int)grid;
@@ -87,12 +88,12 @@ template <int N> struct P32 {
   }
   // P4: the three nonlinear components along y, truncated to the retained ky; combines the x pass's maxima
   static int yfwd(const Xform &x, const float2 *X2, float2 *X1, const Ctl *ctl, const double *pmax, int nblk) {
-    const int lds = N * T * (int)sizeof(double2);
+    const int lds = N * T * (int)sizeof(float2);
     allow_lds(fast::kf_yfwd<N, T, float2>, lds);
     const unsigned grid = (unsigned)(ntile * N);
     const int nred = pmax && nblk > 0 ? (int)std::min<long long>(grid, nblk / fast::PassNT<N>::v + 1) : 0;
     hipLaunchKernelGGL((fast::kf_yfwd<N, T, float2>), dim3(grid, 3u), dim3(fast::PassNT<N>::v), lds, x.st, X2, X1, x.g,
-                       x.twN, ctl, (size_t)0, pmax, nblk, nred);
+                       (const float2 *)x.twN32, ctl, (size_t)0, pmax, nblk, nred);
     LAUNCH_CHECK();
     return NSL_OK;
   }
@@ -103,11 +104,13 @@ template <int N> struct P32 {
   template <int STAGE>
   static void zu_stage(const Xform &x, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,
                        double *partials, Ctl *ctl) {
-    const int lds = N * ZC::T * (int)sizeof(double2);
+    // LDS: the transform's N T float2, which also holds the block sums' 8 NT doubles (N T = 8 NT)
+    static_assert(N * ZC::T * sizeof(float2) >= 8 * ZC::NT * sizeof(double), "the sums' scratch fits");
+    const int lds = N * ZC::T * (int)sizeof(float2);
     const double inv = 1.0 / ((double)N * N * N);
     allow_lds(fast::kf_zfwd_upd<N, STAGE, 0, float2>, lds);
     hipLaunchKernelGGL((fast::kf_zfwd_upd<N, STAGE, 0, float2>), dim3((unsigned)zu_blocks), dim3(ZC::NT), lds,
-                       x.st, X1, U, S, ACC, TS, x.g, x.twN, inv, partials, ctl, (size_t)0);
+                       x.st, X1, U, S, ACC, TS, x.g, (const float2 *)x.twN32, inv, partials, ctl, (size_t)0);
   }
   // *nsum: the sum partials k_finalize adds
   static int zu(const Xform &x, int stage, const float2 *X1, const float2 *U, float2 *S, float2 *ACC, float2 *TS,

@@ -46,8 +46,8 @@ struct StepRecDev {
   double d0[8];
 };
 
-// Storage element of the spectral arrays: double2 (FP64 solves) or float2 (FP32 solves: complex64 in HBM, the
-// arithmetic in registers stays FP64). ldc widens a stored value, stc rounds a value to the storage precision, and
+// Storage element of the spectral arrays: double2 (FP64 solves) or float2 (FP32 solves: complex64 in HBM; the update
+// and the sums work in FP64 registers, the transforms in float2). ldc widens a stored value, stc rounds a value to the storage precision, and
 // rnd<E> is the value a store to E followed by a load gives back (the identity for double2).
 __device__ __forceinline__ double2 ldc(const double2 &v) { return v; }
 __device__ __forceinline__ double2 ldc(const float2 &v) { return make_double2((double)v.x, (double)v.y); }
@@ -56,6 +56,16 @@ __device__ __forceinline__ void stc(float2 &d, const double2 &v) { d = make_floa
 template <class E> __device__ __forceinline__ double2 rnd(const double2 &v) {
   if constexpr (std::is_same<E, double2>::value) return v;
   else return make_double2((double)(float)v.x, (double)(float)v.y);
+}
+// The same between a stored element E and a compute value C of any precision (the pass kernels of FP32 solves
+// transform in float2): ldx reads a stored value as C, stx stores a C value as E (rounded when E is narrower).
+template <class C, class E> __device__ __forceinline__ C ldx(const E &v) {
+  if constexpr (std::is_same<C, E>::value) return v;
+  else { C r; r.x = v.x; r.y = v.y; return r; }
+}
+template <class E, class C> __device__ __forceinline__ void stx(E &d, const C &v) {
+  if constexpr (std::is_same<C, E>::value) d = v;
+  else { E r; r.x = v.x; r.y = v.y; d = r; }
 }
 
 // chooseDt (nslab.js:305-310), evaluated with the reference's operation order.

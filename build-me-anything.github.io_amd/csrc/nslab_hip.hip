@@ -154,6 +154,7 @@ struct Xform {
   int fT = 0;
   TuneCfg tune;             // fixed when the transform set is made: every solve of this set runs the same kernels
   double2 *twN = nullptr;   // (cos, sin)(2 pi k / N), k < N
+  float2 *twN32 = nullptr;  // the same rounded to float, for the single-precision transforms of FP32 solves
 };
 
 // Slot batching of a launch: nz slots (gridDim.z), slot buffers sb bytes apart. {1, 0}: one solve.
@@ -323,14 +324,20 @@ static int xform_init(Xform &x, const Geo &g, hipStream_t st) {
     }
     HIP_TRY(hipMalloc(&x.twN, g.N * sizeof(double2)));
     HIP_TRY(hipMemcpy(x.twN, tw.data(), g.N * sizeof(double2), hipMemcpyHostToDevice));
+    std::vector<float2> tw32(g.N);
+    for (int k = 0; k < g.N; k++) tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+    HIP_TRY(hipMalloc(&x.twN32, g.N * sizeof(float2)));
+    HIP_TRY(hipMemcpy(x.twN32, tw32.data(), g.N * sizeof(float2), hipMemcpyHostToDevice));
   }
   return NSL_OK;
 }
 static void xform_free(Xform &x) {
   if (x.hp.twd) (void)hipFree(x.hp.twd);
   if (x.twN) (void)hipFree(x.twN);
+  if (x.twN32) (void)hipFree(x.twN32);
   x.hp.twd = nullptr;
   x.twN = nullptr;
+  x.twN32 = nullptr;
 }
 
 #include "nsl_fastdisp.hpp"

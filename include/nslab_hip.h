@@ -71,11 +71,12 @@ typedef struct {
 int nsl_create(const nsl_opts *opts, nsl_solver **out);
 /* Solver(N, Re, ic, icParams, cfl, dt, xp, fp32=True) of the GPU runner (nslab_gpu.py:56-65): the same solve with
  * every stepped array (state, RK4 registers, each pass's output) held in complex64, half the bytes of nsl_create's
- * FP64 solve; the arithmetic inside each pass stays FP64 in registers, so results carry FP32 rounding at every pass
- * boundary. For N = 32, 48, 64, 96, 128, 192, 256, 384, 512, 1024 (NSL_ERR_ARG otherwise). Never a pool slot. The
- * host views (initial field, nsl_get_state/nsl_set_state, nsl_norms and the diagnostics) allocate an FP64 copy of
- * the state and its transform scratch for the duration of the call, and nsl_spectrum_noalloc refuses an FP32 solve
- * with NSL_ERR_STATE. */
+ * FP64 solve; the transforms and the product u x omega run in single precision as the runner's do, the update
+ * (projection, viscous term, RK4) and the sums in FP64, so results carry FP32 rounding in every pass. For N = 32,
+ * 48, 64, 96, 128, 192, 256, 384, 512, 1024 (NSL_ERR_ARG otherwise). Never a pool slot. The host views (initial
+ * field, nsl_get_state/nsl_set_state, nsl_norms and the diagnostics) allocate an FP64 copy of the state and its
+ * transform scratch for the duration of the call, and nsl_spectrum_noalloc refuses an FP32 solve with
+ * NSL_ERR_STATE. */
 int nsl_create_fp32(const nsl_opts *opts, nsl_solver **out);
 /* 64 or 32: the storage precision of a solve (the runner's fp32 flag, nslab_gpu.py:59). */
 int nsl_precision(const nsl_solver *s);

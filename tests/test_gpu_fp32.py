@@ -1,4 +1,4 @@
-"""FP32 solves (the GPU runner's fp32 mode, nslab_gpu.py:56-65): complex64 storage, FP64 arithmetic in registers.
+"""FP32 solves (the GPU runner's fp32 mode, nslab_gpu.py:56-65): complex64 storage, single-precision transforms.
 
 What pins them:
   * the conversion itself, exactly: nsl_set_state rounds each retained mode to float32 and nsl_get_state returns
