@@ -183,9 +183,21 @@ template <int N, int T> struct Interleaved {
   static __device__ __forceinline__ int at(int n, int t) { return n * T + t; }
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { t = idx % T; bb = idx / T; }
 };
-template <int N, int T> struct LineMajor {   // one pad slot per 8 (per 4: x pass 184 -> 212 us at 256^3; per 16: 187)
-  static constexpr int STRIDE = N + N / 8, SIZE = T * STRIDE;
-  static __device__ __forceinline__ int at(int n, int t) { return t * STRIDE + n + (n >> 3); }
+// SW selects the slot of element n within its line: 0, one pad slot after every 8 (per 4: x pass 184 -> 212 us at
+// 256^3; per 16: 187); 1, none; 2, the low 4 bits of n XORed with bits 4..7; 3, the low 3 bits XORed with bits 3..5;
+// 4, one pad slot after every 16.
+template <int N, int T, int SW = 0> struct LineMajor {
+  static_assert(SW == 0 || SW == 1 || (SW == 2 && N % 16 == 0) || (SW == 3 && N % 8 == 0) || (SW == 4 && N % 16 == 0),
+                "a swizzle permutes whole blocks of the line");
+  static constexpr int STRIDE = SW == 0 ? N + N / 8 : SW == 4 ? N + N / 16 : N, SIZE = T * STRIDE;
+  static __device__ __forceinline__ int slot(int n) {
+    if constexpr (SW == 0) return n + (n >> 3);
+    else if constexpr (SW == 1) return n;
+    else if constexpr (SW == 2) return n ^ ((n >> 4) & 15);
+    else if constexpr (SW == 3) return n ^ ((n >> 3) & 7);
+    else return n + (n >> 4);
+  }
+  static __device__ __forceinline__ int at(int n, int t) { return t * STRIDE + slot(n); }
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { bb = idx % NB; t = idx / NB; }
 };
 

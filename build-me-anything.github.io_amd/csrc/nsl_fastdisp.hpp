@@ -160,6 +160,8 @@ static constexpr bool xp_has(int N, int form) {
 static constexpr bool xs_has(int N) {
   return N == 32 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 || N == 1024;
 }
+// sizes with the one-wave shape of the staged x pass (variant 2: fast::XnlOneWave)
+static constexpr bool xs2_has(int N) { return N == 48 || N == 96 || N == 192 || N == 384; }
 static constexpr bool zs_has(int N) {
   return N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 384 || N == 512 ||
          N == 1024;
@@ -176,8 +178,9 @@ static long long xnl_workgroups(const TuneCfg &c, int cus) {
   switch (c.xpass) {
     case NSL_XP_STAGED:
       if constexpr (xp_has(N, NSL_XP_STAGED)) {
-        const int RPW = c.xs ? fast::XnlStaged<N, 1>::RPW : fast::XnlStaged<N, 0>::RPW;
-        const int WPR = c.xs ? fast::XnlStaged<N, 1>::WPR : fast::XnlStaged<N, 0>::WPR;
+        int RPW = fast::XnlStaged<N, 0>::RPW, WPR = fast::XnlStaged<N, 0>::WPR;
+        if (c.xs == 1) { RPW = fast::XnlStaged<N, 1>::RPW; WPR = fast::XnlStaged<N, 1>::WPR; }
+        if constexpr (xs2_has(N)) if (c.xs == 2) { RPW = fast::XnlStaged<N, 2>::RPW; WPR = fast::XnlStaged<N, 2>::WPR; }
         long long grid = (long long)N * N / RPW;
         // persistent: xwaves waves per CU walk the row groups
         if (c.xwaves > 0) grid = std::min<long long>(grid, std::max(1, cus * c.xwaves / WPR));
@@ -233,8 +236,12 @@ static int fast_xnl_n(const Xform &x, double2 *X2, double *pmax, Ctl *ctl, const
       if constexpr (xp_has(N, NSL_XP_STAGED)) {
         if (c.xs == 0) {
           hipLaunchKernelGGL((fast::kf_xnl_dma<N, 0>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 0>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
-        } else if constexpr (xs_has(N)) {
-          hipLaunchKernelGGL((fast::kf_xnl_dma<N, 1>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 1>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        } else if (c.xs == 1) {
+          if constexpr (xs_has(N))
+            hipLaunchKernelGGL((fast::kf_xnl_dma<N, 1>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 1>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
+        } else {
+          if constexpr (xs2_has(N))
+            hipLaunchKernelGGL((fast::kf_xnl_dma<N, 2>), dim3((unsigned)grid, 1, bt.nz), dim3(fast::XnlStaged<N, 2>::NT), 0, x.st, X2, x.g, x.twN, pmax, ctl, bt.sb);
         }
       }
       break;

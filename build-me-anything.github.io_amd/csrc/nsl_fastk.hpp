@@ -822,9 +822,21 @@ template <> struct XnlSched<256, 1> { using L = RList<4, 16, 4>; };
 template <> struct XnlSched<384, 1> { using L = RList<8, 3, 2, 8>; };
 template <> struct XnlSched<512, 1> { using L = RList<8, 4, 2, 8>; };
 template <> struct XnlSched<1024, 1> { using L = RList<8, 16, 8>; };
+// Variant 2 (N = 48, 96, 192, 384: N/R does not divide 64): the default schedule in one wave per row group, RPW rows
+// of N/R butterflies with the remaining lanes idle in the first and last stages (the largest RPW <= 64/(N/R) that
+// divides N^2), so no stage needs a workgroup barrier and the twiddles stay in registers.
+template <int N> struct XnlOneWave {
+  static constexpr int NB = N / FirstR<typename XnlWave<N>::L>::R;
+  static constexpr bool ok = NB < 64 && 64 % NB != 0;
+  static constexpr int rpw(int r) { return r <= 1 || (N * N) % r == 0 ? r : rpw(r - 1); }
+  static constexpr int RPW = ok ? rpw(64 / NB) : 1;
+};
 template <int N, int V = 0> struct XnlStaged {
   static constexpr int R = FirstR<typename XnlSched<N, V>::L>::R, NB = N / R;
-  static constexpr int NT = NB / gcd_int(NB, 64) * 64, RPW = NT / NB, WPR = NT / 64;
+  static constexpr bool ONE = V == 2;
+  static_assert(!ONE || XnlOneWave<N>::ok, "variant 2 is the one-wave shape of a multi-wave size");
+  static constexpr int NT = ONE ? 64 : NB / gcd_int(NB, 64) * 64, RPW = ONE ? XnlOneWave<N>::RPW : NT / NB;
+  static constexpr int WPR = NT / 64, ACT = NB * RPW;   // ACT: threads with a first- and last-stage butterfly
   // two-wave rows (1024^3): at most 512 / NSL_XNL_BIG_MINW registers; three-wave groups: at most 256
   // (three waves per SIMD, 168 registers, spills 95 at 192^3 and 111 at 384^3)
   static constexpr int MINW = NT == 128 ? NSL_XNL_BIG_MINW : NT > 64 ? 2 : 1;
@@ -851,14 +863,19 @@ template <int N, int S, int R0, int... Rs> struct TwLen<N, S, RList<R0, Rs...>> 
   static constexpr int rest = TwLen<N, S * R0, RList<Rs...>>::v;
   static constexpr int v = mine > rest ? mine : rest;
 };
-template <int N, int V = 0>
+// The staged x pass's LDS layout per N (LineMajor's SW), from the x pass timed alone on every layout: the low 3 bits
+// XORed with bits 3..5 (SW 3) at 48-512 (192^3 120 -> 117 us, 384^3 1046 -> 1021 us, 128^3 28.9 -> 27.8 us, 256^3
+// and 512^3 within the noise), the low 4 bits XORed with bits 4..7 (SW 2) at 1024^3 (16.9 -> 16.4 ms); no padding
+// (SW 1) was 20-30 % slower everywhere.
+template <int N> __host__ __device__ constexpr int xnl_sw() { return N == 1024 ? 2 : N % 8 == 0 ? 3 : 0; }
+template <int N, int V = 0, int SW = xnl_sw<N>()>
 __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
                                                                const double2 *__restrict__ tw,
                                                                double *__restrict__ pmax, Ctl *__restrict__ ctl,
                                                                size_t sb) {
   using C = XnlStaged<N, V>;
-  constexpr int R = C::R, NB = C::NB, NT = C::NT, RPW = C::RPW, WPR = C::WPR;
-  static_assert(NB * RPW == NT, "first and last stage: one butterfly per thread");
+  constexpr int R = C::R, NB = C::NB, NT = C::NT, RPW = C::RPW, WPR = C::WPR, ACT = C::ACT;
+  static_assert(ACT == NT || (C::ONE && ACT < NT), "first and last stage: one butterfly per thread (or idle lanes)");
   using SCH = typename XnlSched<N, V>::L;
   // the forward transforms start from the inverse's last-stage registers: slot j of the radix-R last stage holds
   // element bb + j*N/R, which is what a radix-R first stage reads, so the schedule starts and ends in radix R
@@ -866,7 +883,7 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
   constexpr Geo g = trunc_geo<N>();
   constexpr int hx = g.hx, HP = g.hp;
   constexpr size_t cs = (size_t)N * N * HP;
-  using LY = LineMajor<N, RPW>;
+  using LY = LineMajor<N, RPW, SW>;
   constexpr int CH = (RPW * HP + 63) / 64, FB = CH * 64;   // 1 KB chunks per field of the RPW rows
   __shared__ double2 stg[2 * FB];
   __shared__ double2 xch[LY::SIZE];
@@ -902,15 +919,22 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
   else twc = twl;
   // first stage: radix R0, K0 butterflies per thread (slot (k, j) is element bb + j*NB0 of line t)
   using F0 = FirstR<SCH>;
-  constexpr int R0 = F0::R, NB0 = N / R0, K0 = NB0 * RPW / NT;
-  static_assert(K0 * NT == NB0 * RPW, "the first stage covers the workgroup");
+  constexpr int R0 = F0::R, NB0 = N / R0, TOT0 = NB0 * RPW, K0 = (TOT0 + NT - 1) / NT;
+  static_assert(K0 * NT == TOT0 || C::ONE, "the first stage covers the workgroup");
   int t0[K0], bb0[K0];
 #pragma unroll
   for (int k = 0; k < K0; k++) LY::template split<NB0>(tid + k * NT, t0[k], bb0[k]);
+  const bool act = !C::ONE || tid < ACT;   // this thread owns a last-stage butterfly (all threads unless C::ONE)
   double om2 = 0.0, u2 = 0.0;
   for (int row0 = blockIdx.x * RPW; row0 < N * N; row0 += gridDim.x * RPW) {
   const int next = row0 + gridDim.x * RPW;
   double2 U[3][R];
+  if constexpr (C::ONE) {   // idle lanes keep defined values (their products feed no store and no maximum)
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int j = 0; j < R; j++) U[l][j] = make_double2(0.0, 0.0);
+  }
 #pragma unroll
   for (int l = 0; l < 3; l++) {
     // line l's staging loads have landed: vmcnt(0) in every wave (explicit: the compiler's own wait for loads into
@@ -927,9 +951,10 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
         v[k][j] = make_double2(0.0, 0.0);
         // elements of slot (k, j): bb0 + j*NB0 with bb0 in [k*NT, k*NT + NT) for one row per workgroup, in
         // [0, NB0) otherwise; a slot wholly in the zero band reads nothing
-        constexpr int span = RPW == 1 ? NT : NB0;
+        constexpr int span = RPW == 1 ? (NT < NB0 ? NT : NB0) : NB0;
         const int lo_n = j * NB0 + (RPW == 1 ? k * NT : 0);
         if (lo_n >= hx && lo_n + span - 1 <= N - hx) continue;
+        if (TOT0 % NT != 0 && tid + k * NT >= TOT0) continue;   // idle lane (one-wave shape)
         const int n = bb0[k] + j * NB0;
         const bool lo = n < hx, hi = n > N - hx;
         const int m = t0[k] * HP + (lo ? n : (hi ? N - n : 0));
@@ -948,7 +973,7 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
 #pragma unroll
   for (int j = 0; j < R; j++) {
     const double a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
-    om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c);
+    if (act) { om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c); }
     P[0][j] = make_double2(b * rr - c * q, c * pp - a * rr);
     P[1][j] = make_double2(a * q - b * pp, 0.0);
   }

@@ -25,7 +25,8 @@ static bool tune_valid(int N, const TuneCfg &c) {
   if (c.xpass == NSL_XP_PAIR && !((c.lp == 1 || c.lp == 2 || c.lp == 4 || c.lp == 8) && xp_pair_lp_ok(N, c.lp))) return false;
   if (c.xpass == NSL_XP_STAGED && (c.xwaves < 0 || c.xwaves > 256)) return false;
   if ((c.pt != 0 && c.pt != 1) || (c.pt && alt_T(N) == 0)) return false;
-  if ((c.xs != 0 && c.xs != 1) || (c.xs && !(c.xpass == NSL_XP_STAGED && xs_has(N)))) return false;
+  if (c.xs < 0 || c.xs > 2 || (c.xs && c.xpass != NSL_XP_STAGED)) return false;
+  if ((c.xs == 1 && !xs_has(N)) || (c.xs == 2 && !xs2_has(N))) return false;
   if ((c.zs != 0 && c.zs != 1) || (c.zs && !(c.zfused && zs_has(N)))) return false;
   return true;
 }
@@ -105,7 +106,7 @@ static std::vector<TuneCfg> tune_candidates(int N) {
   }
   std::vector<TuneCfg> out;
   for (int pt : {0, 1})
-    for (int xs : {0, 1})
+    for (int xs : {0, 1, 2})
       for (int zs : {0, 1})
         for (TuneCfg c : base) {
           c.pt = pt; c.xs = xs; c.zs = zs;

@@ -46,7 +46,7 @@ function args() {
 const now = () => Number(process.hrtime.bigint()) / 1e9;
 const key = (c) => c.join(',');
 const describe = (c) => FORM[c[0]] + (c[0] === 1 ? ' x' + c[1] : '') + (c[0] === 4 ? ' ' + (c[2] ? c[2] + '/CU' : 'one group per workgroup') : '') + (c[3] ? ', fused update' : '') +
-  (c[4] ? ', alternative pencil tile' : '') + (c[5] ? ', alternative x-pass schedule' : '') + (c[6] ? ', alternative update schedule' : '');
+  (c[4] ? ', alternative pencil tile' : '') + (c[5] === 1 ? ', alternative x-pass schedule' : c[5] === 2 ? ', one-wave x-pass groups' : '') + (c[6] ? ', alternative update schedule' : '');
 const variants = (c) => (c[4] || 0) + (c[5] || 0) + (c[6] || 0);
 
 // Steps/s of the forms `cfg` at N: TGV, nu = 1/1600, fixed dt; three untimed steps (their energy is the

@@ -257,7 +257,9 @@ int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
  *   zfused: 1 = z-forward fused with the spectral update, 0 = two kernels
  *   ptile : pencil-pass tile width, 0 the default lines per workgroup of N, 1 the alternative (32, 48, 64: 8 lines;
  *           96: 40; 128, 192: 24; no alternative at other sizes)
- *   xsched: radix schedule of the staged x pass, 0 the default, 1 the alternative (xpass 4 only)
+ *   xsched: staged x pass (xpass 4 only): 0 the default radix schedule, 1 the alternative, 2 the default schedule in
+ *           one-wave row groups with idle lanes in the first and last stages (48, 96, 192, 384, whose default groups
+ *           are three waves)
  *   zsched: radix schedule of the fused z-forward, 0 the default, 1 the alternative (zfused 1 only) */
 typedef struct {
   int32_t xpass, lp, xwaves, zfused;

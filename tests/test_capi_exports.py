@@ -75,7 +75,8 @@ def test_tune_table_and_overrides_without_gpu():
     # forms not compiled for a size are refused
     for N, bad in ((96, (3, 0, 0, 0)), (96, (1, 3, 0, 0)), (1024, (1, 1, 0, 0)), (768, (2, 0, 0, 1)), (512, (1, 1, 0, 0)),
                    (256, (4, 0, 16, 1, 1, 0, 0)), (48, (4, 0, 16, 1, 0, 1, 0)), (128, (1, 2, 0, 1, 0, 1, 0)),
-                   (128, (4, 0, 16, 0, 0, 0, 1)), (128, (4, 0, 16, 1, 2, 0, 0))):
+                   (128, (4, 0, 16, 0, 0, 0, 1)), (128, (4, 0, 16, 1, 2, 0, 0)), (256, (4, 0, 16, 1, 0, 2, 0)),
+                   (192, (1, 2, 0, 1, 0, 2, 0)), (384, (4, 0, 16, 1, 0, 3, 0))):
         try:
             capi.tune_set(N, bad)
             raise AssertionError("expected an error for %s at %d" % (bad, N))
@@ -110,6 +111,11 @@ def test_xpass_plan_is_checked_before_the_launch():
                     assert "exceed their buffer" in str(e)
     # the staged form at 1024^3 without persistence: two waves per row, one pair per workgroup, N^2 workgroups
     assert capi.xpass_plan(1024, (4, 0, 0, 1)) == 1024 * 1024
+    # the one-wave shape (xsched 2): rows of N/R butterflies per wave, the most that fit 64 lanes and divide N^2
+    # (192: 24 per row, 2 rows; 384: 48, 1 row; 96: 24, 2 rows; 48: 12, 4 rows, since 5 does not divide 48^2)
+    for N, rows in ((192, 2), (384, 1), (96, 2), (48, 4)):
+        assert capi.xpass_plan(N, (4, 0, 0, 1, 0, 2, 0)) == N * N // rows, N
+        assert capi.xpass_plan(N, (4, 0, 16, 1, 0, 2, 0), cus=256) == min(N * N // rows, 256 * 16), N
     try:
         capi.xpass_plan(96, (3, 0, 0, 0))
         raise AssertionError("expected an error")

@@ -35,7 +35,8 @@ def test_every_form_matches_the_tuned_form(N):
     # the persistent-wave counts share one kernel: check no persistence and 16 per CU besides the tuned form
     forms = [f for f in capi.tune_candidates(N) if f[0] != 4 or f[2] in (0, 16)]
     assert any(f[4] for f in forms) == (N in (32, 48, 64, 96, 128, 192)), N
-    assert any(f[5] for f in forms) == (N != 48) and any(f[6] for f in forms), N
+    assert any(f[5] == 1 for f in forms) == (N != 48) and any(f[6] for f in forms), N
+    assert any(f[5] == 2 for f in forms) == (N in (48, 96, 192, 384)), N
     for cfg in forms:
         if cfg == ref_cfg:
             continue
