@@ -863,11 +863,12 @@ template <int N, int S, int R0, int... Rs> struct TwLen<N, S, RList<R0, Rs...>> 
   static constexpr int rest = TwLen<N, S * R0, RList<Rs...>>::v;
   static constexpr int v = mine > rest ? mine : rest;
 };
-// The staged x pass's LDS layout per N (LineMajor's SW), from the x pass timed alone on every layout: the low 3 bits
-// XORed with bits 3..5 (SW 3) at 48-512 (192^3 120 -> 117 us, 384^3 1046 -> 1021 us, 128^3 28.9 -> 27.8 us, 256^3
-// and 512^3 within the noise), the low 4 bits XORed with bits 4..7 (SW 2) at 1024^3 (16.9 -> 16.4 ms); no padding
-// (SW 1) was 20-30 % slower everywhere.
-template <int N> __host__ __device__ constexpr int xnl_sw() { return N == 1024 ? 2 : N % 8 == 0 ? 3 : 0; }
+// The staged x pass's LDS layout (LineMajor's SW), from the x pass timed alone on every layout, each launch on a
+// fresh copy of the same buffers: the low 3 bits XORed with bits 3..5 (SW 3) is the fastest or tied at every size
+// (256^3 178 -> 171 us, 192^3 120 -> 117 us, 384^3 1046 -> 1021 us, 128^3 28.9 -> 27.8 us, 1024^3 17.2 -> 16.7 ms,
+// 512^3 equal); the 4-bit XOR (SW 2) and one pad slot per 16 (SW 4) fall between, and no padding (SW 1) is 20-30 %
+// slower everywhere.
+template <int N> __host__ __device__ constexpr int xnl_sw() { return N % 8 == 0 ? 3 : 0; }
 template <int N, int V = 0, int SW = xnl_sw<N>()>
 __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
                                                                const double2 *__restrict__ tw,
