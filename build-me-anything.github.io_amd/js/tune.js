@@ -180,9 +180,18 @@ function choose(rows, margin) {
   return cur;
 }
 
+// Rows of the sizes tuned now replace theirs; the rows of sizes not tuned in this run stay as the file has them.
 function writeTable(file, results) {
+  const rows = new Map();
+  if (fs.existsSync(file)) {
+    for (const line of fs.readFileSync(file, 'utf8').split('\n')) {
+      const m = /^\{(\d+),/.exec(line);
+      if (m) rows.set(+m[1], line);
+    }
+  }
+  for (const r of results) rows.set(r.N, `{${r.N}, ${r.pick.concat([0, 0, 0]).slice(0, 7).join(', ')}},   // ${r.pick_what}`);
   const lines = ['// {N, xpass, lp, xwaves, zfused, pt, xs, zs}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4).'];
-  for (const r of results) lines.push(`{${r.N}, ${r.pick.concat([0, 0, 0]).slice(0, 7).join(', ')}},   // ${r.pick_what}`);
+  for (const N of [...rows.keys()].sort((a, b) => a - b)) lines.push(rows.get(N));
   fs.writeFileSync(file, lines.join('\n') + '\n');
 }
 

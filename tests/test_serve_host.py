@@ -59,3 +59,26 @@ console.log(JSON.stringify(SV.sweepSummary(lv)));
 """)
     assert abs(out["slope_omMax"] - 0.5) < 1e-12 and abs(out["slope_eps"] + 0.25) < 1e-12
     assert out["verdict"] == "WARN" and out["Re"] == [707, 1000, 1414]
+
+
+def test_tuner_table_keeps_the_sizes_not_tuned(tmp_path):
+    """js/tune.js rewrites only the rows of the sizes it tuned: a run over a subset of sizes keeps every other row of
+    the table as it was, and the rows come out in order of N."""
+    table = os.path.join(ROOT, "build-me-anything.github.io_amd", "csrc", "nsl_tune_table.inc")
+    f = tmp_path / "table.inc"
+    shutil.copy(table, f)
+    before = [l for l in open(table) if l.startswith("{")]
+    tune = os.path.join(ROOT, "build-me-anything.github.io_amd", "js", "tune.js")
+    res = [{"N": 192, "pick": [4, 0, 8, 1, 0, 2, 0], "pick_what": "staged wave 8/CU (test)"},
+           {"N": 100, "pick": [2, 0, 0, 1], "pick_what": "rows (test)"}]
+    run_js("require(%s).writeTable(%s, %s); console.log('{}');" % (json.dumps(tune), json.dumps(str(f)), json.dumps(res)))
+    after = [l for l in open(f) if l.startswith("{")]
+    assert len(after) == len(before) + 1
+    Ns = [int(l[1:].split(",")[0]) for l in after]
+    assert Ns == sorted(Ns)
+    for l in before:
+        N = int(l[1:].split(",")[0])
+        if N != 192:
+            assert l in after, l
+    assert "{192, 4, 0, 8, 1, 0, 2, 0},   // staged wave 8/CU (test)\n" in after
+    assert "{100, 2, 0, 0, 1, 0, 0, 0},   // rows (test)\n" in after
