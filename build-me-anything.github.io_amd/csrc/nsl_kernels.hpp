@@ -671,6 +671,58 @@ __global__ void k_project(double2 *__restrict__ U, const Geo g) {
   }
 }
 
+// Random solenoidal field, the draws (nslab.js:273-283): mode (kz, ky, kx) of the mask, in the reference's loop order
+// over the masked modes (the compact storage order: cz, then cy, then kx), takes 12 Mulberry32 numbers (3 components
+// x 2 parts x 2 draws, Box-Muller), amplitude sqrt(k^4 exp(-2 (k/k0)^2) / (4 pi k^2)); the modes that do not draw
+// (k = 0, and on kx = 0 the half kz < 0 or kz = 0, ky < 0) are left to the conjugate fill of their partner. Mulberry32's
+// state after n draws is a0 + n * 0x6D2B79F5 (mod 2^32), so each mode starts from its own draw count: with hx kx per
+// row and the kx = 0 mode drawing only for kz > 0 or kz = 0 < ky, a row r = cz * by + cy starts after
+// r (hx - 1) + (rows before r whose kx = 0 mode draws) draws of 12 numbers. dst (3 components of D slots) is zero
+// on entry. One thread per mode (cz, cy, kx < hx).
+// Mulberry32 (nslab.js:161): a = a + 0x6D2B79F5, then a 32-bit hash of a.
+__device__ __forceinline__ double mb32_next_dev(unsigned &a) {
+  a += 0x6D2B79F5u;
+  unsigned t = (a ^ (a >> 15)) * (1u | a);
+  t = (t + (t ^ (t >> 7)) * (61u | t)) ^ t;
+  return (double)(t ^ (t >> 14)) / 4294967296.0;
+}
+__global__ void k_ic_random(double2 *__restrict__ dst, const Geo g, unsigned a0, double k0) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)g.by * g.by * g.hx) return;
+  const int i = (int)(e % g.hx);
+  const long long r = e / g.hx;
+  const int cy = (int)(r % g.by), cz = (int)(r / g.by), N = g.N, kc = g.hy - 1;
+  const int l = full_of(cz, g), j = full_of(cy, g);
+  const double kx = i, ky = kwv(j, N), kz = kwv(l, N);
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  const bool draw0 = kz > 0 || (kz == 0 && ky > 0);   // this row's kx = 0 mode draws
+  if (k2 == 0 || (i == 0 && !draw0)) return;
+  // rows before r whose kx = 0 mode draws: cz = 0 rows with 0 < ky (cy 1..kc), every row of 0 < kz (cz 1..kc)
+  const long long d0 = cz == 0 ? (long long)max(0, min(cy, kc + 1) - 1)
+                                : cz <= kc ? (long long)kc + (long long)(cz - 1) * g.by + cy : (long long)kc + (long long)kc * g.by;
+  const long long before = r * (g.hx - 1) + d0 + (i == 0 ? 0 : (draw0 ? 1 : 0) + (i - 1));
+  unsigned a = a0 + (unsigned)((unsigned long long)before * 12ull * 0x6D2B79F5ull);
+  const double k = sqrt(k2), amp = sqrt(pow(k, 4.0) * exp(-2 * (k / k0) * (k / k0)) / (4 * M_PI * k2));
+  const size_t m = ((size_t)cz * g.by + cy) * g.hp + i;
+  double2 v[3];
+  for (int c = 0; c < 3; c++) {
+    double p[2];
+    for (int part = 0; part < 2; part++) {
+      double ra = mb32_next_dev(a);
+      ra = fmax(ra, 1e-12);
+      const double rb = mb32_next_dev(a);
+      p[part] = amp * (sqrt(-2 * log(ra)) * cos(NSL_TWO_PI * rb));
+    }
+    v[c] = make_double2(p[0], p[1]);
+    dst[(size_t)c * g.D + m] = v[c];
+  }
+  if (i == 0) {   // the conjugate partner (a mode that does not draw: no other thread writes it)
+    const int czm = compact_of((N - l) % N, g), cym = compact_of((N - j) % N, g);
+    const size_t mm = ((size_t)czm * g.by + cym) * g.hp;
+    for (int c = 0; c < 3; c++) dst[(size_t)c * g.D + mm] = make_double2(v[c].x, -v[c].y);
+  }
+}
+
 // Tubes (nslab.js:267-272): W = project(omega-hat); U = (i k x W) / k^2 (k = 0 -> /1); S = project(U).
 __global__ void k_tubes_finish(const double2 *__restrict__ W, double2 *__restrict__ S, const Geo g) {
   const long long D = g.D;

@@ -16,7 +16,6 @@
 #include <mutex>
 #include <set>
 #include <string>
-#include <thread>
 #include <utility>
 #include <vector>
 
@@ -678,91 +677,21 @@ struct AuxView {
   ~AuxView() { aux_release(s); }
 };
 
-// Mulberry32 (nslab.js:161): a = a + 0x6D2B79F5, then a 32-bit hash of a.
-static inline double mb32_next(uint32_t &a) {
-  a += 0x6D2B79F5u;
-  uint32_t t = (a ^ (a >> 15)) * (1u | a);
-  t = (t + (t ^ (t >> 7)) * (61u | t)) ^ t;
-  return (double)(t ^ (t >> 14)) / 4294967296.0;
-}
-
 static inline int full_of_h(int c, const Geo &g) { return c < g.hy ? c : c + g.N - g.by; }
 static inline int compact_of_h(int f, const Geo &g) {
   return f < g.hy ? f : (f >= g.N - g.by + g.hy ? f - (g.N - g.by) : -1);
 }
 static inline double kw_h(int f, int N) { return (double)(f <= N / 2 ? f : f - N); }
 
-// Random solenoidal field (nslab.js:273-286): the PRNG draws follow the reference's loop order over the masked
-// modes, which is the compact storage order. Drawn on the host, then projected and scaled to E0 on the device.
-// Every mode that draws (k != 0, outside the kx = 0 half the conjugate fill covers) takes exactly 12 numbers (3
-// components x 2 parts x 2 draws), and Mulberry32's state after n draws is a0 + n * 0x6D2B79F5 (mod 2^32), so the
-// kz slabs are drawn on several threads, each starting from its slab's draw count: the same numbers as one
-// sequential pass (1024^3: 480 M modes).
-static bool ic_draws(int i, double ky, double kz) { return !(i == 0 && (kz < 0 || (kz == 0 && ky < 0))); }
+// Random solenoidal field (nslab.js:273-286): the draws of every mode on the device (k_ic_random, each mode from its
+// own position in the reference's draw sequence), then projected and scaled to E0.
 static int ic_random(nsl_solver *s, double2 *dst, double seed, double k0, double E0) {
   const Geo &g = s->x.g;
-  const int N = g.N;
-  const uint32_t a0 = (uint32_t)(int32_t)seed;
-  std::vector<double2> h((size_t)3 * g.D, make_double2(0.0, 0.0));
-  // draws before each kz slab
-  std::vector<uint64_t> before(g.by + 1, 0);
-  for (int cz = 0; cz < g.by; cz++) {
-    const double kz = kw_h(full_of_h(cz, g), N);
-    uint64_t n = 0;
-    for (int cy = 0; cy < g.by; cy++) {
-      const double ky = kw_h(full_of_h(cy, g), N);
-      for (int i = 0; i < g.hx; i++) {
-        const double kx = i;
-        if (kx * kx + ky * ky + kz * kz != 0 && ic_draws(i, ky, kz)) n++;
-      }
-    }
-    before[cz + 1] = before[cz] + 12 * n;
-  }
-  auto slab = [&](int cz) {
-    uint32_t a = a0 + (uint32_t)(before[cz] * 0x6D2B79F5ull);
-    const int l = full_of_h(cz, g);
-    const double kz = kw_h(l, N);
-    for (int cy = 0; cy < g.by; cy++) {
-      const int j = full_of_h(cy, g);
-      const double ky = kw_h(j, N);
-      for (int i = 0; i < g.hx; i++) {
-        const double kx = i;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        if (k2 == 0) continue;
-        if (!ic_draws(i, ky, kz)) continue;
-        const double k = sqrt(k2), amp = sqrt(pow(k, 4) * exp(-2 * (k / k0) * (k / k0)) / (4 * M_PI * k2));
-        const size_t m = ((size_t)cz * g.by + cy) * g.hp + i;
-        for (int c = 0; c < 3; c++) {
-          double v[2];
-          for (int part = 0; part < 2; part++) {
-            double ra = mb32_next(a);
-            ra = std::max(ra, 1e-12);
-            const double rb = mb32_next(a);
-            v[part] = amp * (sqrt(-2 * log(ra)) * cos(TWO_PI * rb));
-          }
-          h[(size_t)c * g.D + m] = make_double2(v[0], v[1]);
-        }
-        if (i == 0) {   // the conjugate partner (kz <= 0 side) is skipped by its own iteration: no other writer
-          const int czm = compact_of_h((N - l) % N, g), cym = compact_of_h((N - j) % N, g);
-          const size_t mm = ((size_t)czm * g.by + cym) * g.hp;
-          for (int c = 0; c < 3; c++) {
-            const double2 v = h[(size_t)c * g.D + m];
-            h[(size_t)c * g.D + mm] = make_double2(v.x, -v.y);
-          }
-        }
-      }
-    }
-  };
-  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (nth == 1 || g.by < 64) {
-    for (int cz = 0; cz < g.by; cz++) slab(cz);
-  } else {
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nth; t++)
-      pool.emplace_back([&, t] { for (int cz = t; cz < g.by; cz += nth) slab(cz); });
-    for (auto &th : pool) th.join();
-  }
-  HIP_TRY(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice, s->st));
+  HIP_TRY(hipMemsetAsync(dst, 0, (size_t)3 * g.D * sizeof(double2), s->st));
+  const long long nm = (long long)g.by * g.by * g.hx;
+  hipLaunchKernelGGL(k_ic_random, dim3((unsigned)((nm + NSL_NT - 1) / NSL_NT)), dim3(NSL_NT), 0, s->st, dst, g,
+                     (unsigned)(int32_t)seed, k0);
+  LAUNCH_CHECK();
   hipLaunchKernelGGL(k_project, dim3(sweep_grid(g.D)), dim3(NSL_NT), 0, s->st, dst, g);
   LAUNCH_CHECK();
   double *nrm = (double *)((char *)s->ctl + offsetof(Ctl, nrm));   // device scratch for the three norms

@@ -46,12 +46,13 @@ function stepPools(pools, k) {
 }
 
 /** Advance every pool on its own clock: collect a pool whose chunk has finished and relaunch it while it has steps
- *  left, so small grids are not held back by the chunk time of large ones. Returns true if a pool was collected. */
+ *  left, so small grids are not held back by the chunk time of large ones. A pool on hold (a queued request waits to
+ *  be admitted into it) is collected but not relaunched. Returns true if a pool was collected. */
 function pump(pools, k) {
   let got = false;
   for (const p of pools) {
     if (p.inflight && p.pool.ready()) { p.running = p.pool.collect(); p.inflight = false; got = true; }
-    if (!p.inflight && p.running > 0) { p.pool.launch(k); p.inflight = true; }
+    if (!p.inflight && p.running > 0 && !p.hold) { p.pool.launch(k); p.inflight = true; }
   }
   return got;
 }
@@ -222,11 +223,14 @@ function serve(opts = {}) {
     for (const p of pools) if (p.N === N && p.pool.free - (claimed.get(p) || 0) > 0) return p;
     const slots = poolSlots(N, gbPerPool), bytes = slots * NS.Pool.slotBytes(N);
     if (reserved + bytes > budget) return null;
-    const p = { N, pool: new NS.Pool(N, slots), running: 0, inflight: false };
+    const p = { N, pool: new NS.Pool(N, slots), running: 0, inflight: false, hold: false };
     reserved += p.pool.info().bytes;
     pools.push(p);
     return p;
   }
+  // A member's initial field and first rhs run on its pool's stream, so admitting into a pool with a chunk in flight
+  // would wait for that chunk while no other pool is relaunched. A request whose pools are busy puts them on hold
+  // instead (they finish their chunk and are not relaunched) and is admitted once they are idle.
   function tryAdmit(q) {
     const claimed = new Map(), at = [];
     for (const m of q.members) {
@@ -234,6 +238,10 @@ function serve(opts = {}) {
       if (!p) return false;
       claimed.set(p, (claimed.get(p) || 0) + 1);
       at.push(p);
+    }
+    if (at.some(p => p.inflight)) {
+      for (const p of at) p.hold = true;
+      return false;
     }
     // the last member first (a ladder's finest rung; a sweep's members share grid and IC): its initial max|u| fixes
     // the common end time (the step budget at CFL 0.5 on that grid)
@@ -252,10 +260,16 @@ function serve(opts = {}) {
     return true;
   }
 
+  let admitS = 0;   // host time spent admitting (initial fields and first rhs of the new members)
   while (completed < reqs.length) {
     const t = now();
     while (nextArrival < reqs.length && reqs[nextArrival].arrival <= t) queue.push(reqs[nextArrival++]);
-    for (let i = 0; i < queue.length;) { if (tryAdmit(queue[i])) queue.splice(i, 1); else i++; }
+    for (const p of pools) p.hold = false;   // set again below by the requests still waiting for busy pools
+    for (let i = 0; i < queue.length;) {
+      const a0 = now(), ok = tryAdmit(queue[i]);
+      admitS += now() - a0;
+      if (ok) queue.splice(i, 1); else i++;
+    }
     if (!live.length) {
       if (queue.length) throw new Error(`request ${queue[0].id} does not fit the HBM budget`);
       sleepMs(1000 * (reqs[nextArrival].arrival - now()));
@@ -292,7 +306,7 @@ function serve(opts = {}) {
   return { requests: reqs.length, sweeps, rate: opts.rate || 2, wall_s: wall, requests_per_hour: reqs.length / wall * 3600,
     ladders_per_hour: (reqs.length - sweeps) / wall * 3600, sweeps_per_hour: sweeps / wall * 3600,
     rung_solves_per_hour: solves / wall * 3600, p50_latency_s: quantile(lat, 0.5), p99_latency_s: quantile(lat, 0.99),
-    rung_steps: rungSteps, pools: poolList, reserved_bytes: reserved,
+    rung_steps: rungSteps, pools: poolList, reserved_bytes: reserved, admit_host_s: admitS,
     budget_bytes: budget, results };
 }
 
