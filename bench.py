@@ -426,6 +426,8 @@ def main():
                         "Re log-uniform [707,4000], CFL 0.5, 300 RK4 steps, one hipGraph per pool step",
             "solve_steps_per_s": e["solve_steps_per_s"], "run_s": e["run_s"], "admit_s": e["admit_s"],
             "sequential_s": e.get("sequential_s"), "speedup_vs_sequential": e.get("speedup_vs_sequential"),
+            "sequential_with_setup_s": e.get("sequential_with_setup_s"),
+            "speedup_with_setup": e.get("speedup_with_setup"),
             "pool_equals_single": e.get("pool_equals_single"), "graphed": e["graphed"],
             "b_alg_GBs": sum(n * b_alg_step(N) for N, n in ((32, 64), (48, 32), (64, 24), (96, 8))) * 300
                          / e["run_s"] / 1e9}
