@@ -201,6 +201,10 @@ template <int N, int T, int SW = 0> struct LineMajor {
   template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { bb = idx % NB; t = idx / NB; }
 };
 
+// The thread's index within the NT threads that run a transform: a one-wave transform (NT = 64) may run in each wave
+// of a larger workgroup, every wave on lines of its own.
+template <int NT> __device__ __forceinline__ int xf_tid() { return NT == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x; }
+
 // Twiddle table values of one thread for every stage of a radix schedule, read once. A kernel that runs several
 // transforms in the same layout (the x pass: five per row) reads them before the first transform and keeps them in
 // registers, so no stage waits on a table read. Slot j of stage S holds tw[j * p * S] for the j that twiddles<R>
@@ -218,7 +222,7 @@ template <int N, int T, int NT, class LY, int S, int R0, int... Rs, class C> str
     if constexpr (HAS) {
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int idx = threadIdx.x + k * NT;
+        const int idx = xf_tid<NT>() + k * NT;
         int t, bb;
         LY::template split<NB>(idx, t, bb);
         const int base = (bb / S) * S;
@@ -277,7 +281,7 @@ __device__ __forceinline__ void stage(C *__restrict__ lds, TWS &tw, LD &load, ST
   C v[K][R];
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const int idx = threadIdx.x + k * NT;
+    const int idx = xf_tid<NT>() + k * NT;
     if (TOT % NT == 0 || idx < TOT) {
       int t, bb;
       LY::template split<NB>(idx, t, bb);
@@ -294,7 +298,7 @@ __device__ __forceinline__ void stage(C *__restrict__ lds, TWS &tw, LD &load, ST
   if (!FIRST || SYNC_FIRST) stage_sync<LW, NT>();
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    const int idx = threadIdx.x + k * NT;
+    const int idx = xf_tid<NT>() + k * NT;
     if (TOT % NT == 0 || idx < TOT) {
       int t, bb;
       LY::template split<NB>(idx, t, bb);

@@ -146,7 +146,8 @@ static int fast_zfwd_n(const Xform &x, const double2 *X4, double2 *OUT, int ncom
 
 // x-pass forms compiled for each N. Row pairs: N <= 384, LP row pairs per workgroup with 6*LP*N complex of LDS
 // (at most 160 KB). Rows: N >= 384. One wave per row group (plain or staged): N = 256, 512.
-enum { NSL_XP_PAIR = 1, NSL_XP_ROWS = 2, NSL_XP_WAVE = 3, NSL_XP_STAGED = 4 };
+// NSL_XP_PLANE: the y-inverse, the x pass and the y-forward as one pass, one workgroup per z-plane (fast::kf_plane).
+enum { NSL_XP_PAIR = 1, NSL_XP_ROWS = 2, NSL_XP_WAVE = 3, NSL_XP_STAGED = 4, NSL_XP_PLANE = 5 };
 static constexpr bool xp_pair_lp_ok(int N, int LP) { return N <= 384 && 6 * LP * N * 16 <= 160 * 1024; }
 static constexpr bool xp_has(int N, int form) {
   return form == NSL_XP_PAIR ? N <= 384
@@ -154,6 +155,7 @@ static constexpr bool xp_has(int N, int form) {
        : form == NSL_XP_WAVE ? (N == 256 || N == 512)
        : form == NSL_XP_STAGED ? (N == 32 || N == 48 || N == 64 || N == 96 || N == 128 || N == 192 || N == 256 ||
                                   N == 384 || N == 512 || N == 1024)
+       : form == NSL_XP_PLANE ? fast::plane_has(N)
        : false;
 }
 // sizes with a variant-1 schedule of the staged x pass (XnlSched) and of the fused update (ZuSchedV)
@@ -189,6 +191,9 @@ static long long xnl_workgroups(const TuneCfg &c, int cus) {
       return 0;
     case NSL_XP_WAVE:
       if constexpr (xp_has(N, NSL_XP_WAVE)) return (long long)N * N / (64 / (N / 8));
+      return 0;
+    case NSL_XP_PLANE:   // one workgroup per z-plane; its maxima go to the control block (no pairs in pmax)
+      if constexpr (xp_has(N, NSL_XP_PLANE)) return N;
       return 0;
     case NSL_XP_ROWS:
       if constexpr (xp_has(N, NSL_XP_ROWS)) return (long long)N * N;   // one row per workgroup
@@ -296,12 +301,31 @@ static int fast_dispatch_zinv(const Xform &x, const double2 *U, double2 *X1, int
                               const Fin &fin) {
   NSL_DISPATCH(fast_zinv_n, U, X1, mode, nout, gc, ctl, bt, fin)
 }
+// P2-P4 in one pass (the x-pass form NSL_XP_PLANE): X1's six fields of every z-plane in, X4 (X1's first three
+// fields) out, the maxima into the control block.
+template <int N>
+static int fast_plane_n(const Xform &x, double2 *X1, Ctl *ctl, const Batch &bt) {
+  if constexpr (fast::plane_has(N)) {
+    constexpr int lds = fast::PlaneCfg<N>::LDS_BYTES;
+    allow_lds(fast::kf_plane<N>, lds);
+    hipLaunchKernelGGL((fast::kf_plane<N>), dim3((unsigned)N, 1, bt.nz), dim3(fast::PlaneCfg<N>::NT), lds, x.st, X1, x.g,
+                       x.twN, ctl, bt.sb);
+    LAUNCH_CHECK();
+    return NSL_OK;
+  }
+  return fail(NSL_ERR_STATE, "the plane pass is not compiled for N = " + std::to_string(N));
+}
+static bool fast_has_plane(const Xform &x) { return x.fast && x.tune.xpass == NSL_XP_PLANE; }
+
 static int fast_dispatch_yinv(const Xform &x, const double2 *X1, double2 *X2, int ncomp, const Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_yinv_n, X1, X2, ncomp, ctl, bt)
 }
 static int fast_dispatch_yfwd(const Xform &x, const double2 *X3, double2 *X4, int ncomp, const Ctl *ctl, const Batch &bt,
                               const double *pmax, int nblk) {
   NSL_DISPATCH(fast_yfwd_n, X3, X4, ncomp, ctl, bt, pmax, nblk)
+}
+static int fast_dispatch_plane(const Xform &x, double2 *X1, Ctl *ctl, const Batch &bt) {
+  NSL_DISPATCH(fast_plane_n, X1, ctl, bt)
 }
 static int fast_dispatch_zfwd(const Xform &x, const double2 *X4, double2 *OUT, int ncomp, const Ctl *ctl, const Batch &bt) {
   NSL_DISPATCH(fast_zfwd_n, X4, OUT, ncomp, ctl, bt)

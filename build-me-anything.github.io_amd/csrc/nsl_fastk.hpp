@@ -1030,5 +1030,160 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
   }
 }
 
+// ------------------------------------------------------------------------------------------------ plane pass
+// P2-P4 of one z-plane in one workgroup, for grids whose six y-transformed fields fit the 160 KB of LDS (N = 32:
+// 48 KB, N = 48: 108 KB): the plane's six fields come from X1 and are y-inverted in place in LDS, the x pass runs on
+// the plane's rows there (one wave per group of RPW rows, the staged x pass's shape and arithmetic), and the three
+// products are y-transformed forward into X4. X2 (6 h N^2 complex values written, 9 h N^2 read and 3 h N^2 written
+// again by the three passes) never goes to HBM. X4 is X1's first three fields: each workgroup writes only the plane
+// it read. Maxima: one atomic pair per workgroup into the control block, as the y-forward pass combines them.
+// Lines of NF fields of one plane in LDS, field-major: line t = field t / HP, column t % HP; element n of a line at
+// field * N * HP + n * HP + column, so a field's rows are the x pass's rows.
+template <int N, int HP, int NF> struct PlaneLY {
+  static constexpr int T = NF * HP, SIZE = NF * N * HP;
+  static __device__ __forceinline__ int at(int n, int t) { return (t / HP) * N * HP + n * HP + t % HP; }
+  template <int NB> static __device__ __forceinline__ void split(int idx, int &t, int &bb) { t = idx % T; bb = idx / T; }
+};
+template <int N> struct PlaneCfg {
+  static constexpr Geo g = trunc_geo<N>();
+  // 48^3: 512 threads (one workgroup per CU, limited by LDS: eight waves, 12 row groups of the x pass)
+  static constexpr int HP = g.hp, NT = N == 48 ? 512 : 256, NW = NT / 64;
+  static constexpr int XV = XnlOneWave<N>::ok ? 2 : 0;   // the staged x pass's one-wave shape
+  using XC = XnlStaged<N, XV>;
+  static_assert(XC::NT == 64, "one wave per row group");
+  static constexpr int RPW = XC::RPW;
+  using LX = LineMajor<N, RPW, xnl_sw<N>()>;
+  static_assert(N % RPW == 0, "whole row groups per plane");
+  static constexpr int PLANE = 6 * N * HP, XCH = NW * LX::SIZE;
+  static constexpr int LDS_BYTES = (PLANE + XCH) * (int)sizeof(double2);
+};
+static constexpr bool plane_has(int N) { return N == 32 || N == 48; }
+
+template <int N>
+__global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const Geo, const double2 *__restrict__ tw,
+                                                Ctl *__restrict__ ctl, size_t sb) {
+  using C = PlaneCfg<N>;
+  using XC = typename C::XC;
+  constexpr Geo g = trunc_geo<N>();
+  constexpr int HP = C::HP, NT = C::NT, RPW = C::RPW, R = XC::R, NB = XC::NB, hx = g.hx;
+  static_assert(C::LDS_BYTES <= 160 * 1024, "the plane fits the LDS");
+  X1 = at_slot(X1, sb); ctl = at_slot(ctl, sb);
+  if (ctl->active == 0) return;
+  extern __shared__ double2 lds_raw[];
+  double2 *P = lds_raw;                                          // [field][y][kx], 6 fields
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, z = blockIdx.x;
+  double2 *xch = lds_raw + C::PLANE + wave * C::LX::SIZE;       // this wave's exchange buffer
+  const size_t plane = (size_t)g.by * g.hp;
+  // 1. y-inverse of the six fields: X1[o][z][cy][kx] -> P[o][y][kx] (zero band and padding columns read as zeros)
+  {
+    using LY = PlaneLY<N, HP, 6>;
+    auto load = [&](int n, int t, int, int j) -> double2 {
+      if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
+      const int cy = compact_of(n, g), o = t / HP, kx = t % HP;
+      return (cy >= 0 && kx < hx) ? X1[((size_t)o * N + z) * plane + (size_t)cy * g.hp + kx] : make_double2(0.0, 0.0);
+    };
+    auto store = [&](int n, int t, double2 v) { P[LY::at(n, t)] = v; };
+    fft<N, LY::T, NT, 1, false, LY>(P, tw, load, store);
+  }
+  __syncthreads();
+  // 2. the x pass on the plane's rows (kf_xnl_dma's packing, product and unpacking; its inputs and outputs in P)
+  using LY = typename C::LX;
+  using SCH = typename XnlWave<N>::L;
+  TwCache<N, RPW, 64, LY, SCH> twc;
+  twc.load(tw);
+  using F0 = FirstR<SCH>;
+  constexpr int R0 = F0::R, NB0 = N / R0, TOT0 = NB0 * RPW, K0 = (TOT0 + 63) / 64;
+  static_assert(R0 == R && K0 == 1, "one first-stage butterfly per lane");
+  int t0, bb0;
+  LY::template split<NB0>(lane, t0, bb0);
+  const bool act = lane < NB * RPW;
+  double om2 = 0.0, u2 = 0.0;
+  for (int row0 = wave * RPW; row0 < N; row0 += C::NW * RPW) {
+    double2 U[3][R];
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+#pragma unroll
+      for (int j = 0; j < R; j++) U[l][j] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      double2 v[R0];
+#pragma unroll
+      for (int j = 0; j < R0; j++) {
+        v[j] = make_double2(0.0, 0.0);
+        const int lo_n = j * NB0;
+        if (lo_n >= hx && lo_n + NB0 - 1 <= N - hx) continue;
+        if (lane >= TOT0) continue;
+        const int n = bb0 + j * NB0;
+        const bool lo = n < hx, hi = n > N - hx;
+        const int m = (row0 + t0) * HP + (lo ? n : (hi ? N - n : 0));
+        const double2 A = P[(2 * l) * N * HP + m], B = P[(2 * l + 1) * N * HP + m];
+        const double2 d = lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
+        v[j] = lo || hi ? d : make_double2(0.0, 0.0);
+      }
+      auto ldr = [&](int, int, int, int j) -> double2 { return v[j]; };
+      auto keep = [&](int, int, double2 w, int, int j) { U[l][j] = w; };
+      fft_sched_lds<N, RPW, 64, 1, false, LY, SCH>(xch, twc, ldr, keep);
+    }
+    double2 Pr[2][R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const double a = U[0][j].x, b = U[0][j].y, c = U[1][j].x, pp = U[1][j].y, q = U[2][j].x, rr = U[2][j].y;
+      if (act) { om2 = fmax(om2, pp * pp + q * q + rr * rr); u2 = fmax(u2, a * a + b * b + c * c); }
+      Pr[0][j] = make_double2(b * rr - c * q, c * pp - a * rr);
+      Pr[1][j] = make_double2(a * q - b * pp, 0.0);
+    }
+    // the wave's reads of its rows in P are complete (each stage waits for its LDS operations), so the products
+    // overwrite fields 0..2 of the same rows
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      auto ld2 = [&](int, int, int, int j) -> double2 { return Pr[l][j]; };
+      auto st2 = [&](int n, int t, double2 w) {
+        if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
+      };
+      fft_sched_lds<N, RPW, 64, -1, false, LY, SCH>(xch, twc, ld2, st2);
+      stage_sync<true, 64>();
+#pragma unroll
+      for (int i = 0; i < (RPW * HP + 63) / 64; i++) {
+        const int e = lane + i * 64;
+        if (e >= RPW * HP) continue;
+        const int t = e / HP, k = e - t * HP;
+        double2 o0 = make_double2(0.0, 0.0), o1 = o0;
+        if (k < hx) {
+          const double2 Zk = xch[LY::at(k, t)], Zc = xch[LY::at((N - k) % N, t)];
+          if (l == 0) {
+            const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+            o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+            o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+          } else {
+            o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+          }
+        }
+        const int m = (row0 + t) * HP + k;
+        if (l == 0) {
+          P[m] = o0;
+          P[N * HP + m] = o1;
+        } else {
+          P[2 * N * HP + m] = o0;
+        }
+      }
+      stage_sync<true, 64>();   // the next transform overwrites xch
+    }
+  }
+  // maxima of the plane into the control block (xch of the first wave as the reduction's scratch)
+  block_atomic_max2<NT>(om2, u2, &ctl->om2, &ctl->u2, reinterpret_cast<double *>(lds_raw + C::PLANE));
+  // (block_atomic_max2 ends with the workgroup past a barrier: every row's products are in P)
+  __syncthreads();
+  // 3. y-forward of the three products: P[c][y][kx] -> X4[c][z][cy][kx] = X1[c][z][cy][kx], retained ky only
+  {
+    using LY3 = PlaneLY<N, HP, 3>;
+    auto load = [&](int n, int t) -> double2 { return P[LY3::at(n, t)]; };
+    auto store = [&](int n, int t, double2 v) {
+      const int cy = compact_of(n, g);
+      if (cy >= 0) X1[((size_t)(t / HP) * N + z) * plane + (size_t)cy * g.hp + t % HP] = v;
+    };
+    fft<N, LY3::T, NT, -1, true, LY3>(P, tw, load, store);
+  }
+}
+
 }  // namespace fast
 }  // namespace nsl

@@ -99,6 +99,7 @@ static std::vector<TuneCfg> tune_candidates(int N) {
     }
     { TuneCfg c; c.xpass = NSL_XP_ROWS; c.zfused = z; if (tune_valid(N, c)) base.push_back(c); }
     { TuneCfg c; c.xpass = NSL_XP_WAVE; c.zfused = z; if (tune_valid(N, c)) base.push_back(c); }
+    { TuneCfg c; c.xpass = NSL_XP_PLANE; c.zfused = z; if (tune_valid(N, c)) base.push_back(c); }
     for (int w : {0, 8, 16, 32, 64}) {
       TuneCfg c; c.xpass = NSL_XP_STAGED; c.xwaves = w; c.zfused = z;
       if (tune_valid(N, c)) base.push_back(c);

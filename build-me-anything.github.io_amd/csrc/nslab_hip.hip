@@ -535,10 +535,14 @@ static int rhs_enqueue(nsl_solver *s, bool onS, int stage, int fmode, StepRecDev
   const Batch &bt = s->bt;
   double2 *U = onS ? s->S : s->T;
   TRY(prof_begin(s, NSL_PROF_ZINV)); TRY(zinv(x, U, s->X1, 1, 6, 0, s->ctl, bt, fin)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
-  int nred = 0;
-  TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, &nred, bt)); TRY(prof_end(s));
-  TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt, s->pmax, nred)); TRY(prof_end(s));
+  if (fast_has_plane(x)) {   // y-inverse, x pass and y-forward in one pass (timed as the x pass)
+    TRY(prof_begin(s, NSL_PROF_XNL)); TRY(fast_dispatch_plane(x, s->X1, s->ctl, bt)); TRY(prof_end(s));
+  } else {
+    TRY(prof_begin(s, NSL_PROF_YINV)); TRY(yinv(x, s->X1, s->X2, 6, s->ctl, bt)); TRY(prof_end(s));
+    int nred = 0;
+    TRY(prof_begin(s, NSL_PROF_XNL)); TRY(xnl(x, s->X2, s->pmax, s->ctl, &nred, bt)); TRY(prof_end(s));
+    TRY(prof_begin(s, NSL_PROF_YFWD)); TRY(yfwd(x, s->X2, s->X1, 3, s->ctl, bt, s->pmax, nred)); TRY(prof_end(s));
+  }
   if (fast_has_zu(x)) {
     // z-forward and update in one pass: the nonlinear term stays on chip (timed as the update of this stage)
     TRY(prof_begin(s, stage == 1 ? NSL_PROF_UPDATE1 : (stage == 4 ? NSL_PROF_UPDATE4 : NSL_PROF_UPDATE23)));

@@ -22,7 +22,7 @@ const fs = require('fs');
 const path = require('path');
 const addon = require(path.join(__dirname, '..', 'lib', 'nslab_napi.node'));
 
-const FORM = { 1: 'row pairs', 2: 'rows', 3: 'wave', 4: 'staged wave' };
+const FORM = { 1: 'row pairs', 2: 'rows', 3: 'wave', 4: 'staged wave', 5: 'plane pass' };
 const REC = 21;   // doubles per step record of addon.run: t dt step ebal zbal last[8] d0[8]
 
 function args() {
@@ -190,7 +190,7 @@ function writeTable(file, results) {
     }
   }
   for (const r of results) rows.set(r.N, `{${r.N}, ${r.pick.concat([0, 0, 0]).slice(0, 7).join(', ')}},   // ${r.pick_what}`);
-  const lines = ['// {N, xpass, lp, xwaves, zfused, pt, xs, zs}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4).'];
+  const lines = ['// {N, xpass, lp, xwaves, zfused, pt, xs, zs}: written by js/tune.js (NSL_XP_PAIR 1, ROWS 2, WAVE 3, STAGED 4, PLANE 5).'];
   for (const N of [...rows.keys()].sort((a, b) => a - b)) lines.push(rows.get(N));
   fs.writeFileSync(file, lines.join('\n') + '\n');
 }

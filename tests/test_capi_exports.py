@@ -111,6 +111,13 @@ def test_xpass_plan_is_checked_before_the_launch():
                     assert "exceed their buffer" in str(e)
     # the staged form at 1024^3 without persistence: two waves per row, one pair per workgroup, N^2 workgroups
     assert capi.xpass_plan(1024, (4, 0, 0, 1)) == 1024 * 1024
+    # the plane pass (xpass 5): one workgroup per z-plane, maxima into the control block; compiled at 32 and 48 only
+    assert capi.xpass_plan(32, (5, 0, 0, 1)) == 32 and capi.xpass_plan(48, (5, 0, 0, 0)) == 48
+    try:
+        capi.xpass_plan(64, (5, 0, 0, 1))
+        raise AssertionError("expected an error")
+    except capi.NslError as e:
+        assert "not compiled" in str(e)
     # the one-wave shape (xsched 2): rows of N/R butterflies per wave, the most that fit 64 lanes and divide N^2
     # (192: 24 per row, 2 rows; 384: 48, 1 row; 96: 24, 2 rows; 48: 12, 4 rows, since 5 does not divide 48^2)
     for N, rows in ((192, 2), (384, 1), (96, 2), (48, 4)):
