@@ -1046,8 +1046,11 @@ template <int N, int HP, int NF> struct PlaneLY {
 };
 template <int N> struct PlaneCfg {
   static constexpr Geo g = trunc_geo<N>();
-  // 48^3: 512 threads (one workgroup per CU, limited by LDS: eight waves, 12 row groups of the x pass)
-  static constexpr int HP = g.hp, NT = N == 48 ? 512 : 256, NW = NT / 64;
+  // 48^3: 512 threads (one workgroup per CU, limited by LDS: eight waves, 12 row groups of the x pass). 64^3: rows of
+  // the hx retained columns only (135 KB for the plane, and three waves' exchange buffers in the rest of the 160 KB),
+  // 192 threads, and the y transforms one field at a time (one butterfly per thread and stage).
+  static constexpr int HP = N == 64 ? g.hx : g.hp, NT = N == 48 ? 512 : N == 64 ? 192 : 256, NW = NT / 64;
+  static constexpr bool PER_FIELD = N == 64;
   static constexpr int XV = XnlOneWave<N>::ok ? 2 : 0;   // the staged x pass's one-wave shape
   using XC = XnlStaged<N, XV>;
   static_assert(XC::NT == 64, "one wave per row group");
@@ -1057,7 +1060,7 @@ template <int N> struct PlaneCfg {
   static constexpr int PLANE = 6 * N * HP, XCH = NW * LX::SIZE;
   static constexpr int LDS_BYTES = (PLANE + XCH) * (int)sizeof(double2);
 };
-static constexpr bool plane_has(int N) { return N == 32 || N == 48; }
+static constexpr bool plane_has(int N) { return N == 32 || N == 48 || N == 64; }
 
 template <int N>
 __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const Geo, const double2 *__restrict__ tw,
@@ -1075,7 +1078,7 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
   double2 *xch = lds_raw + C::PLANE + wave * C::LX::SIZE;       // this wave's exchange buffer
   const size_t plane = (size_t)g.by * g.hp;
   // 1. y-inverse of the six fields: X1[o][z][cy][kx] -> P[o][y][kx] (zero band and padding columns read as zeros)
-  {
+  if constexpr (!C::PER_FIELD) {
     using LY = PlaneLY<N, HP, 6>;
     auto load = [&](int n, int t, int, int j) -> double2 {
       if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
@@ -1084,6 +1087,18 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
     };
     auto store = [&](int n, int t, double2 v) { P[LY::at(n, t)] = v; };
     fft<N, LY::T, NT, 1, false, LY>(P, tw, load, store);
+  } else {
+    using LY = PlaneLY<N, HP, 1>;
+#pragma unroll 1
+    for (int o = 0; o < 6; o++) {   // each field's transform writes only its own rows of P
+      auto load = [&](int n, int t, int, int j) -> double2 {
+        if (zero_slot<N>(j)) return make_double2(0.0, 0.0);
+        const int cy = compact_of(n, g);
+        return (cy >= 0 && t < hx) ? X1[((size_t)o * N + z) * plane + (size_t)cy * g.hp + t] : make_double2(0.0, 0.0);
+      };
+      auto store = [&](int n, int t, double2 v) { P[o * N * HP + LY::at(n, t)] = v; };
+      fft<N, LY::T, NT, 1, false, LY>(P + o * N * HP, tw, load, store);
+    }
   }
   __syncthreads();
   // 2. the x pass on the plane's rows (kf_xnl_dma's packing, product and unpacking; its inputs and outputs in P)
@@ -1174,7 +1189,7 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
   // (block_atomic_max2 ends with the workgroup past a barrier: every row's products are in P)
   __syncthreads();
   // 3. y-forward of the three products: P[c][y][kx] -> X4[c][z][cy][kx] = X1[c][z][cy][kx], retained ky only
-  {
+  if constexpr (!C::PER_FIELD) {
     using LY3 = PlaneLY<N, HP, 3>;
     auto load = [&](int n, int t) -> double2 { return P[LY3::at(n, t)]; };
     auto store = [&](int n, int t, double2 v) {
@@ -1182,6 +1197,18 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
       if (cy >= 0) X1[((size_t)(t / HP) * N + z) * plane + (size_t)cy * g.hp + t % HP] = v;
     };
     fft<N, LY3::T, NT, -1, true, LY3>(P, tw, load, store);
+  } else {   // rows of the hx retained columns: the padding columns of X4 keep the zeros the z-inverse wrote there
+    using LY = PlaneLY<N, HP, 1>;
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+      double2 *Pc = P + c * N * HP;
+      auto load = [&](int n, int t) -> double2 { return Pc[LY::at(n, t)]; };
+      auto store = [&](int n, int t, double2 v) {
+        const int cy = compact_of(n, g);
+        if (cy >= 0) X1[((size_t)c * N + z) * plane + (size_t)cy * g.hp + t] = v;
+      };
+      fft<N, LY::T, NT, -1, true, LY>(Pc, tw, load, store);
+    }
   }
 }
 

@@ -252,7 +252,7 @@ int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
  * alone and in any pool.
  *   xpass : x-pass form, 1 row pairs (16x16-style schedules, lp row pairs per workgroup), 2 one row per workgroup,
  *           3 one wave per row group (8x8x8 at 512, 8x4x8 at 256), 4 the same with its input staged through LDS,
- *           5 the plane pass: y-inverse, x pass and y-forward in one workgroup per z-plane (32, 48)
+ *           5 the plane pass: y-inverse, x pass and y-forward in one workgroup per z-plane (32, 48, 64)
  *   lp    : row pairs per workgroup of form 1 (1, 2, 4, 8)
  *   xwaves: persistent waves per CU of form 4 (0: one row group per workgroup)
  *   zfused: 1 = z-forward fused with the spectral update, 0 = two kernels

@@ -111,10 +111,11 @@ def test_xpass_plan_is_checked_before_the_launch():
                     assert "exceed their buffer" in str(e)
     # the staged form at 1024^3 without persistence: two waves per row, one pair per workgroup, N^2 workgroups
     assert capi.xpass_plan(1024, (4, 0, 0, 1)) == 1024 * 1024
-    # the plane pass (xpass 5): one workgroup per z-plane, maxima into the control block; compiled at 32 and 48 only
+    # the plane pass (xpass 5): one workgroup per z-plane, maxima into the control block; compiled at 32, 48 and 64
     assert capi.xpass_plan(32, (5, 0, 0, 1)) == 32 and capi.xpass_plan(48, (5, 0, 0, 0)) == 48
+    assert capi.xpass_plan(64, (5, 0, 0, 1)) == 64
     try:
-        capi.xpass_plan(64, (5, 0, 0, 1))
+        capi.xpass_plan(96, (5, 0, 0, 1))
         raise AssertionError("expected an error")
     except capi.NslError as e:
         assert "not compiled" in str(e)
