@@ -450,7 +450,8 @@ def main():
                         % sv["requests"],
             "ladders_per_hour": sv["ladders_per_hour"], "rung_solves_per_hour": sv["rung_solves_per_hour"],
             "p50_latency_s": sv["p50_latency_s"], "p99_latency_s": sv["p99_latency_s"], "wall_s": sv["wall_s"],
-            "admit_host_s": sv.get("admit_host_s"), "sequential_estimate_s": sv["sequential_estimate_s"],
+            "admit_host_s": sv.get("admit_host_s"), "warm_start_s": sv.get("warm_start_s"),
+            "sequential_estimate_s": sv["sequential_estimate_s"],
             "throughput_vs_sequential": sv["throughput_vs_sequential"],
             "sequential_sample_s": sv.get("sequential_sample_s"),
             "sequential_sample_estimate_s": sv.get("sequential_sample_estimate_s"),
