@@ -246,10 +246,9 @@ int nsl_spectral_inverse(int32_t N, const double *re, const double *im, double *
 /* factorize(N) (nslab.js:28-35): radices into f[0..return), or NSL_ERR_ARG with the reference's message. */
 int nsl_factorize(int32_t N, int32_t *f, int32_t cap);
 
-/* ---- Kernel forms per grid size (autotuner). No counterpart in the reference: it has one code path; the
- * north star asks for tile and radix schedules picked per N by a project-written autotuner (js/tune.js). A
- * solver's forms depend on N only and are fixed when it (or its pool) is created, so a solve gives the same bits
- * alone and in any pool.
+/* ---- Kernel forms per grid size (autotuner). No counterpart in the reference: it has one code path. Tile widths
+ * and radix schedules are picked per N by the project's autotuner (js/tune.js). A solver's forms depend on N only
+ * and are fixed when it (or its pool) is created, so a solve gives the same bits alone and in any pool.
  *   xpass : x-pass form, 1 row pairs (16x16-style schedules, lp row pairs per workgroup), 2 one row per workgroup,
  *           3 one wave per row group (8x8x8 at 512, 8x4x8 at 256), 4 the same with its input staged through LDS,
  *           5 the plane pass: y-inverse, x pass and y-forward in one workgroup per z-plane (32, 48, 64)
