@@ -1032,7 +1032,8 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
 
 // ------------------------------------------------------------------------------------------------ plane pass
 // P2-P4 of one z-plane in one workgroup, for grids whose six y-transformed fields fit the 160 KB of LDS (N = 32:
-// 48 KB, N = 48: 108 KB): the plane's six fields come from X1 and are y-inverted in place in LDS, the x pass runs on
+// 48 KB, N = 48: 108 KB, N = 64: 135 KB in rows of the retained columns): the plane's six fields come from X1 and are
+// y-inverted in place in LDS, the x pass runs on
 // the plane's rows there (one wave per group of RPW rows, the staged x pass's shape and arithmetic), and the three
 // products are y-transformed forward into X4. X2 (6 h N^2 complex values written, 9 h N^2 read and 3 h N^2 written
 // again by the three passes) never goes to HBM. X4 is X1's first three fields: each workgroup writes only the plane
