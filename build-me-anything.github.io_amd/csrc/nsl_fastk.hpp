@@ -869,7 +869,13 @@ template <int N, int S, int R0, int... Rs> struct TwLen<N, S, RList<R0, Rs...>> 
 // 512^3 equal); the 4-bit XOR (SW 2) and one pad slot per 16 (SW 4) fall between, and no padding (SW 1) is 20-30 %
 // slower everywhere.
 template <int N> __host__ __device__ constexpr int xnl_sw() { return N % 8 == 0 ? 3 : 0; }
-template <int N, int V = 0, int SW = xnl_sw<N>()>
+// One-wave groups unpack the forward transforms' outputs straight from the last stage's registers, the partner mode
+// N - k taken across the wave (ds_bpermute), instead of a round trip through LDS (bit-identical; timed alone: 256^3
+// 176.6 -> 167.9 us, 512^3 1379 -> 1352, 192^3 107.9 -> 101.3, 384^3 727 -> 680, 128^3 29.2 -> 27.9, 64^3 9.8 -> 9.3).
+#ifndef NSL_XNL_DIRECT
+#define NSL_XNL_DIRECT 1
+#endif
+template <int N, int V = 0, int SW = xnl_sw<N>(), bool DO = NSL_XNL_DIRECT != 0>
 __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) void kf_xnl_dma(double2 *__restrict__ X2, const Geo,
                                                                const double2 *__restrict__ tw,
                                                                double *__restrict__ pmax, Ctl *__restrict__ ctl,
@@ -981,6 +987,53 @@ __global__ __launch_bounds__((XnlStaged<N, V>::NT), (XnlStaged<N, V>::MINW)) voi
 #pragma unroll
   for (int l = 0; l < 2; l++) {
     auto ld2 = [&](int, int, int, int j) -> double2 { return P[l][j]; };
+    if constexpr (DO && WPR == 1) {
+      // the unpacking straight from the last stage's registers: slot j of this lane holds mode k = bb + j NB of row t,
+      // and its partner N - k sits in lane t NB + (NB - bb) % NB, slot R - 1 - j (bb != 0), or in this lane, slot
+      // (R - j) % R (bb == 0); the partner's value comes across the wave (ds_bpermute), the stores go out per slot
+      double2 Z[R];
+      if constexpr (C::ONE) {   // idle lanes keep defined values (no active lane reads them)
+#pragma unroll
+        for (int j = 0; j < R; j++) Z[j] = make_double2(0.0, 0.0);
+      }
+      auto keepZ = [&](int, int, double2 w, int, int j) { Z[j] = w; };
+      fft_sched_lds<N, RPW, NT, -1, false, LY, SCH>(xch, twc, ld2, keepZ);
+      const int bbL = lane % NB, tL = lane / NB, src = (tL * NB + (NB - bbL) % NB) * 4;
+      auto bperm = [&](double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)(b & 0xffffffffll));
+        const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)((unsigned long long)b >> 32));
+        return __longlong_as_double(((long long)(unsigned)hi << 32) | (long long)(unsigned)lo);
+      };
+      constexpr int JOUT = (HP + NB - 1) / NB;   // slots holding a stored column k < HP
+      const size_t base = (size_t)(row0 + tL) * HP;
+#pragma unroll
+      for (int j = 0; j < JOUT; j++) {
+        const double2 w = Z[R - 1 - j];
+        double2 Zc = make_double2(bperm(w.x), bperm(w.y));
+        if (bbL == 0) Zc = Z[(R - j) % R];
+        const int k = bbL + j * NB;
+        if (!act || k >= HP) continue;
+        const double2 Zk = Z[j];
+        double2 o0 = make_double2(0.0, 0.0), o1 = o0;
+        if (k < hx) {
+          if (l == 0) {
+            const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
+            o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
+            o1 = make_double2(0.5 * (zi - ci), -0.5 * (zr - cr));
+          } else {
+            o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
+          }
+        }
+        if (l == 0) {
+          stc(X2[base + k], o0);
+          stc(X2[cs + base + k], o1);
+        } else {
+          stc(X2[2 * cs + base + k], o0);
+        }
+      }
+      continue;
+    }
     auto st2 = [&](int n, int t, double2 w) {
       if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
     };
@@ -1150,22 +1203,33 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
     }
     // the wave's reads of its rows in P are complete (each stage waits for its LDS operations), so the products
     // overwrite fields 0..2 of the same rows
+    // unpacking straight from the last stage's registers, the partner mode N - k across the wave (as in kf_xnl_dma)
+    const int bbL = lane % NB, tL = lane / NB, src = (tL * NB + (NB - bbL) % NB) * 4;
+    auto bperm = [&](double v) {
+      const long long b = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)(b & 0xffffffffll));
+      const int hi = __builtin_amdgcn_ds_bpermute(src, (int)(unsigned)((unsigned long long)b >> 32));
+      return __longlong_as_double(((long long)(unsigned)hi << 32) | (long long)(unsigned)lo);
+    };
 #pragma unroll
     for (int l = 0; l < 2; l++) {
       auto ld2 = [&](int, int, int, int j) -> double2 { return Pr[l][j]; };
-      auto st2 = [&](int n, int t, double2 w) {
-        if (n < hx || n > N - hx) xch[LY::at(n, t)] = w;
-      };
-      fft_sched_lds<N, RPW, 64, -1, false, LY, SCH>(xch, twc, ld2, st2);
-      stage_sync<true, 64>();
+      double2 Z[R];
 #pragma unroll
-      for (int i = 0; i < (RPW * HP + 63) / 64; i++) {
-        const int e = lane + i * 64;
-        if (e >= RPW * HP) continue;
-        const int t = e / HP, k = e - t * HP;
+      for (int j = 0; j < R; j++) Z[j] = make_double2(0.0, 0.0);
+      auto keepZ = [&](int, int, double2 w, int, int j) { Z[j] = w; };
+      fft_sched_lds<N, RPW, 64, -1, false, LY, SCH>(xch, twc, ld2, keepZ);
+      constexpr int JOUT = (HP + NB - 1) / NB;   // slots holding a stored column k < HP
+#pragma unroll
+      for (int j = 0; j < JOUT; j++) {
+        const double2 w = Z[R - 1 - j];
+        double2 Zc = make_double2(bperm(w.x), bperm(w.y));
+        if (bbL == 0) Zc = Z[(R - j) % R];
+        const int k = bbL + j * NB;
+        if (!act || k >= HP) continue;
+        const double2 Zk = Z[j];
         double2 o0 = make_double2(0.0, 0.0), o1 = o0;
         if (k < hx) {
-          const double2 Zk = xch[LY::at(k, t)], Zc = xch[LY::at((N - k) % N, t)];
           if (l == 0) {
             const double zr = Zk.x, zi = Zk.y, cr = Zc.x, ci = -Zc.y;
             o0 = make_double2(0.5 * (zr + cr), 0.5 * (zi + ci));
@@ -1174,7 +1238,7 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
             o0 = make_double2(0.5 * (Zk.x + Zc.x), 0.5 * (Zk.y - Zc.y));
           }
         }
-        const int m = (row0 + t) * HP + k;
+        const int m = (row0 + tL) * HP + k;
         if (l == 0) {
           P[m] = o0;
           P[N * HP + m] = o1;
@@ -1182,7 +1246,6 @@ __global__ __launch_bounds__(PlaneCfg<N>::NT) void kf_plane(double2 *X1, const G
           P[2 * N * HP + m] = o0;
         }
       }
-      stage_sync<true, 64>();   // the next transform overwrites xch
     }
   }
   // maxima of the plane into the control block (xch of the first wave as the reduction's scratch)
