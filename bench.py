@@ -329,7 +329,7 @@ def main():
                     help="launch one device-to-device copy of this many MiB first (PMC counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each solve runs on one device; "
+        print("bench.py measures one GPU: the solver has no multi-GPU path (each solve runs on one device; "
               "ensembles shard by solve). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch

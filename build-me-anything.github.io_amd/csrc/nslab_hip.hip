@@ -312,7 +312,7 @@ static int xinv(const Xform &x, const double2 *X, double *phys, int ncomp) {
 static int xform_init(Xform &x, const Geo &g, hipStream_t st) {
   x.g = g;
   x.st = st;
-  if (!tiling_for(g.N, x.tl)) return fail(NSL_ERR_ARG, "grid size " + std::to_string(g.N) + " exceeds the LDS tiling of this build (N <= 1024)");
+  if (!tiling_for(g.N, x.tl)) return fail(NSL_ERR_ARG, "grid size " + std::to_string(g.N) + " exceeds the LDS tiling of this library (N <= 1024)");
   TRY(plan_create(g.N, x.hp));
   // the size-specialised kernels hard-code the 2/3-truncated geometry; untruncated transforms use the generic ones
   if (g.by != g.N && fast_setup(x)) {

@@ -2,7 +2,7 @@
 
 Sources (research/nslab/<run>/final.json of the reference checkout, written by
 pocket-wind-tunnel/gpu/nslab_gpu.py:529-533): the tube-pair IC with the default parameters (nslab.js:262-272), CFL
-0.4, adaptive dt, at the grids this build has size-specialised kernels for:
+0.4, adaptive dt, at the grids the HIP library has size-specialised kernels for:
   * FP32 runs (the runner's --fp32 mode, complex64 throughout): expl-tubes-Re707-N128-fp32-gpu,
     expl-tubes-Re1000-N192-fp32-gpu, expl-tubes-Re707-N256-fp32-gpu;
   * FP64 runs: tubes-Re2000-N256-gpu, tubes-Re1000-N384-gpu.

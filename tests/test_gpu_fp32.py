@@ -3,7 +3,7 @@
 What pins them:
   * the conversion itself, exactly: nsl_set_state rounds each retained mode to float32 and nsl_get_state returns
     exactly those rounded values;
-  * this build's FP64 path (itself pinned to the oracle at 1e-13): at every size with an FP32 path, an FP32 solve
+  * the library's FP64 path (itself pinned to the oracle at 1e-13): at every size with an FP32 path, an FP32 solve
     differs from the FP64 solve of the same seeded field by what complex64 storage implies, more than 1e-9 (so the
     FP32 buffers are really in use) and less than 1e-5 relative after 10 steps (a few float32 roundings per pass);
   * the reference's own FP32 runs (tests/golden/archive_series.json: the first 40 steps of the tube runs the runner
@@ -13,7 +13,7 @@ What pins them:
     ~eps32 kc |u| (the archived FP32 runs record 3.6e-7 to 4.8e-7 at t = 0), FP64 ~1e-15; at 256^3 the cutoff
     pile-up of the initial tubes is ungraded in FP32 (E(k8) on the float32 noise floor; the archived 0.1.2 FP32 run
     graded that noise as 2.10).
-And the reference's FP64 tube runs at 256^3 and 384^3 (same fixture), against this build's FP64 path, within 1e-11
+And the reference's FP64 tube runs at 256^3 and 384^3 (same fixture), against the library's FP64 path, within 1e-11
 relative over 40 steps: the headline size against the reference's own archived output."""
 import json
 import os
